@@ -1,0 +1,78 @@
+/*
+ * spme_ext.h -- extension entry points of the MI355X SPME engine.
+ *
+ * The reference has no counterpart for these (its only surface is spme.h);
+ * they expose what the drop-in keeps internal:
+ *   - precision (FP64 or FP32 mesh path) and a batch width of right-hand
+ *     sides that share one P build (the reference loops RHS serially,
+ *     old-version/spme.c:631-702, re-spreading each);
+ *   - device-resident inputs/outputs on a caller's HIP stream (hipStream_t
+ *     passed as void*), so no host copies sit in the timed path;
+ *   - the real-space, overlap and self parts, giving the full RPY M.f.
+ * All functions return 0 on success and a negative code on error; the
+ * message is available from spme_last_error().  Plain C types only.
+ */
+#ifndef SPME_EXT_H_
+#define SPME_EXT_H_
+
+#include "spme.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { SPME_FP64 = 0, SPME_FP32 = 1 };
+
+/* Parts of M to apply (bit mask). */
+enum { SPME_RECIP = 1, SPME_REAL = 2, SPME_SELF = 4, SPME_FULL = 7 };
+
+typedef struct spme_params {
+  double xi;       /* Ewald splitting parameter */
+  int dim;         /* mesh points per side, K */
+  int porder;      /* B-spline order p: 4, 6 or 8 */
+  int np;          /* number of particles N */
+  double box_size; /* cube side L */
+  double rc;       /* real-space cutoff (used by SPME_REAL) */
+  int precision;   /* SPME_FP64 or SPME_FP32 (mesh, spread, FFT, interpolate) */
+  int max_rhs;     /* largest batch of right-hand sides per apply */
+} spme_params_t;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *spme_last_error(void);
+
+int spme_engine_create(const spme_params_t *prm, spme_t **out);
+int spme_engine_destroy(spme_t *spme);
+
+/* Build the interpolation (P) data from positions (N x 3 FP64).  pos is a
+ * device pointer if on_device != 0, else host memory.  Asynchronous on
+ * stream (NULL: the engine's own stream) for device input. */
+int spme_engine_set_positions(spme_t *spme, const double *pos, int on_device, void *stream);
+
+/* out[r*ldout + 3i + c] <- beta*out + alpha*(M f_r)[3i + c], r < nrhs,
+ * for the parts in `parts`.  in/out are device pointers of the engine's
+ * precision (float for SPME_FP32, double for SPME_FP64).  Asynchronous on
+ * stream (NULL: the engine's own stream). */
+int spme_engine_apply(spme_t *spme, int nrhs, double alpha, const void *in, long ldin,
+                      double beta, void *out, long ldout, int parts, void *stream);
+
+/* Layout and sizes, for tests and benchmarks. */
+typedef struct spme_info {
+  int K, P, N, precision, max_rhs;
+  int RS;              /* reals per mesh row (padded) */
+  long comp_stride;    /* reals per mesh component */
+  double L, xi, rc;
+  long mesh_bytes;     /* device bytes of the mesh */
+  void *stream;        /* the engine's own stream */
+} spme_info_t;
+int spme_engine_info(const spme_t *spme, spme_info_t *info);
+
+/* Copy the current mesh (max_rhs*3 components, real view, RS-padded rows)
+ * to host memory; for tests of the intermediate stages. */
+int spme_engine_debug_stage(spme_t *spme, int stage, int nrhs, const void *in, long ldin,
+                            void *host_mesh);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPME_EXT_H_ */

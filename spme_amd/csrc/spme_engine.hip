@@ -1,0 +1,545 @@
+// Host engine and C ABI of the MI355X SPME mobility product.
+//
+// Drop-in surface: include/spme.h (old-version/spme.h:85-101).
+// Extensions:      include/spme_ext.h.
+//
+// One engine owns one HIP stream and all device buffers; calls on one engine
+// are not re-entrant (the mesh is shared scratch), separate engines are
+// independent.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/spme.h"
+#include "../../include/spme_ext.h"
+#include "spme_kernels.h"
+#include "spme_launch.h"
+#include "spme_real.h"
+
+using namespace spme;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(-2, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                  __LINE__);                                                             \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// B-spline tables from the closed form (the reference stores them as
+// literals for p = 4, 6, old-version/spme.c:14-40).  M_p is the cardinal
+// B-spline on [0, p]; W[x][y] is the coefficient of g^y in
+// M_p(g + 1/2 + p - 1 - x), g in [-1/2, 1/2).  Every intermediate is an
+// integer multiple of 2^-(p-1) below 2^53, so the sums are exact and each
+// entry is the correctly rounded quotient.
+
+double binom(int n, int k) {
+  double r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return std::round(r);
+}
+
+double factorial(int n) {
+  double r = 1;
+  for (int i = 2; i <= n; ++i) r *= i;
+  return r;
+}
+
+// M_p(k) for integer k, times (p-1)!.
+double euler_num(int p, int k) {
+  double s = 0;
+  for (int j = 0; j <= k; ++j) {
+    if (k - j <= 0) continue;
+    s += ((j & 1) ? -1.0 : 1.0) * binom(p, j) * std::pow(double(k - j), p - 1);
+  }
+  return s;
+}
+
+void make_W(int p, double *W) {
+  const double fac = factorial(p - 1);
+  for (int x = 0; x < p; ++x) {
+    double coef[8] = {0};
+    for (int j = 0; j <= p - 1 - x; ++j) {
+      const double a = p - 0.5 - x - j;  // (g + a)^(p-1)
+      const double sj = ((j & 1) ? -1.0 : 1.0) * binom(p, j);
+      for (int y = 0; y < p; ++y)
+        coef[y] += sj * binom(p - 1, y) * std::pow(a, p - 1 - y);
+    }
+    for (int y = 0; y < p; ++y) W[x * p + y] = coef[y] / fac;
+  }
+}
+
+// |b(m)|^2 = 1/|sum_{i<p-1} M_p(i+1) e^{i phi i}|^2, phi = 2 pi m/K with m in
+// FFT order (old-version/spme.c:109-126; only the modulus enters lm2).
+void make_bsq(int K, int p, std::vector<double> &bsq) {
+  const double fac = factorial(p - 1);
+  bsq.resize(K);
+  for (int x = 0; x < K; ++x) {
+    const int m = x <= K / 2 ? x : x - K;
+    const double phi = 2.0 * M_PI * m / K;
+    double dr = 0, di = 0;
+    for (int i = 0; i < p - 1; ++i) {
+      const double s = euler_num(p, i + 1) / fac;
+      dr += s * std::cos(i * phi);
+      di += s * std::sin(i * phi);
+    }
+    bsq[x] = 1.0 / (dr * dr + di * di);
+  }
+}
+
+// psi(k) of old-version/spme.c:43-60 with a_i = a_j = 1.
+double psi(double k, double xi) {
+  const double kx = k / xi, kx2 = kx * kx, k2 = k * k;
+  return 6.0 * M_PI * std::exp(-kx2 / 4.0) * (1.0 - k2 / 3.0) / k2 *
+         (1.0 + kx2 * (0.25 + kx2 / 8.0));
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+struct _spme_t {
+  spme_params_t prm;
+  int K, P, N, prec, max_rhs, Kh, Khp, RS;
+  long comp_stride;  // reals
+  size_t tsz;        // bytes per real
+  hipStream_t stream;
+  int device;
+  bool have_pos;
+  // mesh and tables
+  void *grid;  // 3*max_rhs components
+  void *tab;   // influence table [K][K][Khp] in mesh precision
+  void *tw;    // twiddles exp(-2 pi i t/K), K complex
+  // P data
+  int nkeys;
+  double *pos;  // device positions (N x 3)
+  int *base_tmp, *key_tmp, *counts, *cursor, *boff, *perm, *pbase, *err;
+  double *w_tmp;
+  void *pw;  // sorted weights [N][3][P]
+  void *fs;  // sorted forces [N][3*max_rhs]
+  // real space
+  int nc;
+  int *r_key, *r_counts, *r_cursor, *r_off, *r_perm;
+  double *r_pos;
+  void *r_fs;
+  // drop-in staging (FP64 host API)
+  double *h_in, *h_out;
+};
+
+namespace {
+
+template <typename T>
+int upload_tables(_spme_t *e) {
+  const int K = e->K;
+  std::vector<double> bsq;
+  make_bsq(K, e->P, bsq);
+  const double L = e->prm.box_size, V = L * L * L, xi = e->prm.xi;
+  const double dk = 2.0 * M_PI / L;
+  std::vector<T> tab((size_t)K * K * e->Khp, T(0));
+  for (int z = 0; z < K; ++z) {
+    const double kz = (z <= K / 2 ? z : z - K) * dk;
+    for (int y = 0; y < K; ++y) {
+      const double ky = (y <= K / 2 ? y : y - K) * dk;
+      for (int x = 0; x <= K / 2; ++x) {
+        const double kx = x * dk;
+        const double kk = std::sqrt(kx * kx + ky * ky + kz * kz);
+        double v = 0.0;
+        if (kk > 0.0) v = psi(kk, xi) * bsq[x] * bsq[y] * bsq[z] / V;
+        tab[((size_t)z * K + y) * e->Khp + x] = T(v);
+      }
+    }
+  }
+  HIP_TRY(hipMalloc(&e->tab, tab.size() * sizeof(T)));
+  HIP_TRY(hipMemcpy(e->tab, tab.data(), tab.size() * sizeof(T), hipMemcpyHostToDevice));
+  std::vector<T> tw(2 * K);
+  for (int t = 0; t < K; ++t) {
+    const double a = -2.0 * M_PI * t / K;
+    tw[2 * t] = T(std::cos(a));
+    tw[2 * t + 1] = T(std::sin(a));
+  }
+  HIP_TRY(hipMalloc(&e->tw, tw.size() * sizeof(T)));
+  HIP_TRY(hipMemcpy(e->tw, tw.data(), tw.size() * sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int validate(const spme_params_t *p) {
+  if (!p) return fail(-1, "null parameters");
+  if (!(p->porder == 4 || p->porder == 6 || p->porder == 8))
+    return fail(-1, "porder must be 4, 6 or 8 (got %d)", p->porder);
+  if (!fft_size_supported(p->dim))
+    return fail(-1, "dim %d not supported (smooth sizes 16..256 of the form 2^a 3^b, "
+                    "multiple of 8)", p->dim);
+  if (4 * p->porder >= p->dim)
+    return fail(-1, "dim must be larger than 4 * porder (old-version/main.c:205)");
+  if (p->np <= 0) return fail(-1, "np must be positive");
+  if (!(p->box_size > 0.0) || !std::isfinite(p->box_size))
+    return fail(-1, "box_size must be positive");
+  if (!(p->xi > 0.0) || !std::isfinite(p->xi)) return fail(-1, "xi must be positive");
+  if (p->precision != SPME_FP64 && p->precision != SPME_FP32)
+    return fail(-1, "precision must be SPME_FP64 or SPME_FP32");
+  if (p->max_rhs < 1 || p->max_rhs > 16) return fail(-1, "max_rhs must be in [1, 16]");
+  return 0;
+}
+
+void free_all(_spme_t *e) {
+  void *ptrs[] = {e->grid, e->tab, e->tw, e->pos, e->base_tmp, e->key_tmp, e->counts,
+                  e->cursor, e->boff, e->perm, e->pbase, e->err, e->w_tmp, e->pw, e->fs,
+                  e->r_key, e->r_counts, e->r_cursor, e->r_off, e->r_perm, e->r_pos,
+                  e->r_fs, e->h_in, e->h_out};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+}
+
+int engine_create(const spme_params_t *prm, _spme_t **out) {
+  if (int rc = validate(prm)) return rc;
+  _spme_t *e = new _spme_t();
+  std::memset(e, 0, sizeof(*e));
+  e->prm = *prm;
+  e->K = prm->dim;
+  e->P = prm->porder;
+  e->N = prm->np;
+  e->prec = prm->precision;
+  e->max_rhs = prm->max_rhs;
+  e->Kh = e->K / 2 + 1;
+  e->Khp = (e->Kh + 7) / 8 * 8;
+  e->RS = 2 * e->Khp;
+  e->comp_stride = (long)e->K * e->K * e->RS;
+  e->tsz = e->prec == SPME_FP32 ? 4 : 8;
+  const int K = e->K, N = e->N, C = 3 * e->max_rhs;
+  auto bail = [&](int rc) {
+    free_all(e);
+    delete e;
+    return rc;
+  };
+#define E_TRY(expr)                                                                      \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return bail(fail(-2, "%s failed: %s", #expr, hipGetErrorString(e_)));              \
+  } while (0)
+  E_TRY(hipGetDevice(&e->device));
+  E_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  E_TRY(hipMalloc(&e->grid, (size_t)C * e->comp_stride * e->tsz));
+  int rc = e->prec == SPME_FP32 ? upload_tables<float>(e) : upload_tables<double>(e);
+  if (rc) return bail(rc);
+  e->nkeys = (K / 8) * (K / 8) * K;
+  E_TRY(hipMalloc(&e->pos, sizeof(double) * 3 * N));
+  E_TRY(hipMalloc(&e->base_tmp, sizeof(int) * N));
+  E_TRY(hipMalloc(&e->key_tmp, sizeof(int) * N));
+  E_TRY(hipMalloc(&e->counts, sizeof(int) * e->nkeys));
+  E_TRY(hipMalloc(&e->cursor, sizeof(int) * e->nkeys));
+  E_TRY(hipMalloc(&e->boff, sizeof(int) * (e->nkeys + 1)));
+  E_TRY(hipMalloc(&e->perm, sizeof(int) * N));
+  E_TRY(hipMalloc(&e->pbase, sizeof(int) * N));
+  E_TRY(hipMalloc(&e->err, sizeof(int)));
+  E_TRY(hipMalloc(&e->w_tmp, sizeof(double) * 3 * e->P * N));
+  E_TRY(hipMalloc(&e->pw, e->tsz * 3 * e->P * N));
+  E_TRY(hipMalloc(&e->fs, e->tsz * (size_t)C * N));
+  const double L = prm->box_size;
+  // Real-space cell list only when a cutoff is given (the drop-in surface is
+  // reciprocal-only).  Cells are at least max(rc, 2) wide so that every pair
+  // within the cutoff or overlapping lies in adjacent cells.
+  const double cell = prm->rc > 2.0 ? prm->rc : 2.0;
+  e->nc = prm->rc > 0.0 ? int(std::floor(L / cell)) : 0;
+  if (e->nc > 128) e->nc = 128;
+  if (e->nc >= 3) {
+    const int nc3 = e->nc * e->nc * e->nc;
+    E_TRY(hipMalloc(&e->r_key, sizeof(int) * N));
+    E_TRY(hipMalloc(&e->r_counts, sizeof(int) * nc3));
+    E_TRY(hipMalloc(&e->r_cursor, sizeof(int) * nc3));
+    E_TRY(hipMalloc(&e->r_off, sizeof(int) * (nc3 + 1)));
+    E_TRY(hipMalloc(&e->r_perm, sizeof(int) * N));
+    E_TRY(hipMalloc(&e->r_pos, sizeof(double) * 3 * N));
+    E_TRY(hipMalloc(&e->r_fs, e->tsz * (size_t)C * N));
+  }
+#undef E_TRY
+  *out = e;
+  return 0;
+}
+
+template <typename T>
+int build_P(_spme_t *e, hipStream_t st) {
+  const int N = e->N, K = e->K, P = e->P;
+  WTable W;
+  make_W(P, W.w);
+  HIP_TRY(hipMemsetAsync(e->counts, 0, sizeof(int) * e->nkeys, st));
+  HIP_TRY(hipMemsetAsync(e->cursor, 0, sizeof(int) * e->nkeys, st));
+  HIP_TRY(hipMemsetAsync(e->err, 0, sizeof(int), st));
+  launch_pbuild(e->pos, N, e->prm.box_size, K, P, W, 8, 8, e->base_tmp, e->w_tmp, e->key_tmp,
+                e->counts, e->err, st);
+  launch_scan(e->counts, e->boff, e->nkeys, st);
+  launch_pscatter(e->key_tmp, e->boff, e->cursor, e->perm, N, st);
+  launch_bucket_sort(e->boff, e->perm, e->nkeys, st);
+  launch_pgather<T>(e->perm, e->base_tmp, e->w_tmp, e->pbase, (T *)e->pw, N, 3 * P, st);
+  if (e->nc >= 3) {
+    const int nc3 = e->nc * e->nc * e->nc;
+    HIP_TRY(hipMemsetAsync(e->r_counts, 0, sizeof(int) * nc3, st));
+    HIP_TRY(hipMemsetAsync(e->r_cursor, 0, sizeof(int) * nc3, st));
+    launch_cell_key(e->pos, N, e->prm.box_size, e->nc, e->r_key, e->r_counts, st);
+    launch_scan(e->r_counts, e->r_off, nc3, st);
+    launch_pscatter(e->r_key, e->r_off, e->r_cursor, e->r_perm, N, st);
+    launch_bucket_sort(e->r_off, e->r_perm, nc3, st);
+    launch_cell_gather(e->r_perm, e->pos, e->r_pos, N, st);
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int set_positions(_spme_t *e, const double *pos, int on_device, hipStream_t st) {
+  if (!pos) return fail(-1, "positions are NULL");
+  HIP_TRY(hipMemcpyAsync(e->pos, pos, sizeof(double) * 3 * e->N,
+                         on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+  int rc = e->prec == SPME_FP32 ? build_P<float>(e, st) : build_P<double>(e, st);
+  if (rc) return rc;
+  int herr = 0;
+  HIP_TRY(hipMemcpyAsync(&herr, e->err, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (herr) return fail(-1, "positions must be finite and within [-L, 2L)");
+  e->have_pos = true;
+  return 0;
+}
+
+int pick_cc(int C, int P) {
+  if (P < 8 && C % 12 == 0) return 12;
+  return C % 6 == 0 ? 6 : 3;
+}
+
+template <typename T>
+int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double beta, T *out,
+            long ldout, int parts, hipStream_t st, int stop_after) {
+  const int N = e->N, C = 3 * nrhs;
+  if (parts & SPME_RECIP) {
+    launch_permute_in<T>(e->perm, in, ldin, (T *)e->fs, N, C, st);
+    SpreadArgs<T> sa{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, C, pick_cc(C, e->P), e->P,
+                     (T *)e->grid, e->comp_stride, e->RS, e->K,
+                     e->K >= 64 ? e->K / 4 : e->K / 2};
+    if (launch_spread<T>(sa, st)) return fail(-1, "spread: unsupported configuration");
+    if (stop_after == 1) return 0;
+    FftArgs<T> fa{(T *)e->grid, e->comp_stride, e->RS, e->K, C, (const T *)e->tab,
+                  T(2.0 * M_PI / e->prm.box_size), (const cpx<T> *)e->tw};
+    if (launch_fft_influence<T>(fa, st)) return fail(-1, "fft: unsupported size");
+    if (stop_after == 2) return 0;
+    InterpArgs<T> ia{e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid, e->comp_stride,
+                     e->RS, e->K, N, C, e->P, T(alpha), T(beta), out, ldout};
+    if (launch_interp<T>(ia, st)) return fail(-1, "interpolate: unsupported configuration");
+  }
+  if (parts & (SPME_REAL | SPME_SELF)) {
+    if (e->nc < 3)
+      return fail(-1, "real-space part needs box_size >= 3*max(rc, 2) (cell list)");
+    launch_permute_in<T>(e->r_perm, in, ldin, (T *)e->r_fs, N, C, st);
+    const double xi = e->prm.xi;
+    const double self = (parts & SPME_SELF)
+                            ? 1.0 - 6.0 * xi / std::sqrt(M_PI) +
+                                  40.0 * xi * xi * xi / (3.0 * std::sqrt(M_PI))
+                            : 0.0;
+    const double rc2 = (parts & SPME_REAL) ? e->prm.rc * e->prm.rc : 0.0;
+    RealArgs<T> ra{e->r_pos, e->r_perm, e->r_off, e->nc, e->prm.box_size, xi, rc2, self,
+                   (const T *)e->r_fs, N, C, alpha, beta, (parts & SPME_RECIP) ? 1 : 0,
+                   out, ldout};
+    if (!(parts & SPME_REAL)) {
+      // self only: no pairs at all (rc2 = 0 still adds overlap; disable it)
+      return fail(-1, "SPME_SELF without SPME_REAL is not supported");
+    }
+    if (launch_real<T>(ra, st)) return fail(-1, "real space: unsupported batch width");
+  }
+  HIP_TRY(hipGetLastError());
+  return 0;
+}
+
+int apply(_spme_t *e, int nrhs, double alpha, const void *in, long ldin, double beta,
+          void *out, long ldout, int parts, hipStream_t st, int stop_after = 0) {
+  if (!e->have_pos) return fail(-1, "positions have not been set");
+  if (nrhs < 1 || nrhs > e->max_rhs)
+    return fail(-1, "nrhs must be in [1, %d] (got %d)", e->max_rhs, nrhs);
+  if (ldin < 3L * e->N || ldout < 3L * e->N) return fail(-1, "ldin/ldout must be >= 3*np");
+  if (!in || !out) return fail(-1, "null vector");
+  if (parts == 0 || (parts & ~SPME_FULL)) return fail(-1, "bad parts mask %d", parts);
+  if (e->prec == SPME_FP32)
+    return apply_t<float>(e, nrhs, alpha, (const float *)in, ldin, beta, (float *)out, ldout,
+                          parts, st, stop_after);
+  return apply_t<double>(e, nrhs, alpha, (const double *)in, ldin, beta, (double *)out, ldout,
+                         parts, st, stop_after);
+}
+
+void die(const char *where) {
+  fprintf(stderr, "%s: %s\n", where, g_err.c_str());
+  abort();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+extern "C" {
+
+const char *spme_last_error(void) { return g_err.c_str(); }
+
+int spme_engine_create(const spme_params_t *prm, spme_t **out) {
+  if (!out) return fail(-1, "null output pointer");
+  g_err.clear();
+  return engine_create(prm, out);
+}
+
+int spme_engine_destroy(spme_t *spme) {
+  if (!spme) return 0;
+  (void)hipStreamSynchronize(spme->stream);
+  free_all(spme);
+  delete spme;
+  return 0;
+}
+
+int spme_engine_set_positions(spme_t *spme, const double *pos, int on_device, void *stream) {
+  if (!spme) return fail(-1, "null engine");
+  hipStream_t st = stream ? (hipStream_t)stream : spme->stream;
+  return set_positions(spme, pos, on_device, st);
+}
+
+int spme_engine_apply(spme_t *spme, int nrhs, double alpha, const void *in, long ldin,
+                      double beta, void *out, long ldout, int parts, void *stream) {
+  if (!spme) return fail(-1, "null engine");
+  hipStream_t st = stream ? (hipStream_t)stream : spme->stream;
+  return apply(spme, nrhs, alpha, in, ldin, beta, out, ldout, parts, st);
+}
+
+int spme_engine_info(const spme_t *e, spme_info_t *info) {
+  if (!e || !info) return fail(-1, "null argument");
+  info->K = e->K;
+  info->P = e->P;
+  info->N = e->N;
+  info->precision = e->prec;
+  info->max_rhs = e->max_rhs;
+  info->RS = e->RS;
+  info->comp_stride = e->comp_stride;
+  info->L = e->prm.box_size;
+  info->xi = e->prm.xi;
+  info->rc = e->prm.rc;
+  info->mesh_bytes = (long)(3 * e->max_rhs) * e->comp_stride * (long)e->tsz;
+  info->stream = e->stream;
+  return 0;
+}
+
+// stage 1: mesh after spreading; stage 2: after FFT, influence, inverse FFT.
+int spme_engine_debug_stage(spme_t *e, int stage, int nrhs, const void *in, long ldin,
+                            void *host_mesh) {
+  if (!e) return fail(-1, "null engine");
+  if (stage != 1 && stage != 2) return fail(-1, "stage must be 1 or 2");
+  int rc = apply(e, nrhs, 1.0, in, ldin, 0.0, (void *)in, ldin, SPME_RECIP, e->stream, stage);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(host_mesh, e->grid, (size_t)3 * nrhs * e->comp_stride * e->tsz,
+                         hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  return 0;
+}
+
+// --- drop-in (include/spme.h) ---------------------------------------------
+
+void cutoff2xi(double rmax, double ewald_eps, double box_size, double *_xi) {
+  if (!(rmax > 0.0) || !(ewald_eps > 0.0) || !(ewald_eps < 1.0) || !_xi) {
+    fail(-1, "cutoff2xi: need rmax > 0, 0 < ewald_eps < 1");
+    die("cutoff2xi");
+  }
+  (void)box_size;
+  // Largest |eigenvalue| of M_real(rmax) (F and F+G) decreases with xi once
+  // xi*rmax > ~1; bisect on xi for the value ewald_eps.
+  auto tail = [&](double xi) {
+    const double r = rmax, r2 = r * r, xr = xi * r;
+    const double e = std::exp(-xr * xr) / std::sqrt(M_PI), ec = std::erfc(xr);
+    const double x3 = xi * xi * xi, x5 = x3 * xi * xi, x7 = x5 * xi * xi;
+    const double F = ec * (0.75 / r + 0.5 / (r2 * r)) +
+                     e * (4 * x7 * r2 * r2 + 3 * x3 * r2 - 20 * x5 * r2 - 4.5 * xi +
+                          14 * x3 + xi / r2);
+    const double G = ec * (0.75 / r - 1.5 / (r2 * r)) +
+                     e * (-4 * x7 * r2 * r2 - 3 * x3 * r2 + 16 * x5 * r2 + 1.5 * xi -
+                          2 * x3 - 3 * xi / r2);
+    return std::fmax(std::fabs(F), std::fabs(F + G));
+  };
+  double lo = 1.0 / rmax, hi = 50.0 / rmax;
+  for (int it = 0; it < 200; ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (tail(mid) > ewald_eps) lo = mid;
+    else hi = mid;
+  }
+  *_xi = hi;
+}
+
+void create_spme_engine(double xi, int dim, int porder, int np, double box_size, int nthreads,
+                        spme_t **_spme) {
+  (void)nthreads;  // host work here is not threaded; the GPU does the rest
+  spme_params_t p{};
+  p.xi = xi;
+  p.dim = dim;
+  p.porder = porder;
+  p.np = np;
+  p.box_size = box_size;
+  p.rc = 0.0;
+  p.precision = SPME_FP64;
+  p.max_rhs = 1;
+  if (!_spme || spme_engine_create(&p, _spme)) die("create_spme_engine");
+  if (hipMalloc(&(*_spme)->h_in, sizeof(double) * 3 * np) != hipSuccess ||
+      hipMalloc(&(*_spme)->h_out, sizeof(double) * 3 * np) != hipSuccess) {
+    fail(-2, "hipMalloc of staging vectors failed");
+    die("create_spme_engine");
+  }
+}
+
+void destroy_spme_engine(spme_t *spme) { spme_engine_destroy(spme); }
+
+void compute_spme(spme_t *spme, double *pos, int nrhs, double alpha, double *vec_in, int ldin,
+                  double beta, double *vec_out, int ldout) {
+  if (!spme) {
+    fail(-1, "null engine");
+    die("compute_spme");
+  }
+  hipStream_t st = spme->stream;
+  if (pos && set_positions(spme, pos, 0, st)) die("compute_spme");
+  if (!spme->have_pos) {
+    fail(-1, "the first call must pass positions");
+    die("compute_spme");
+  }
+  const size_t n3 = 3 * (size_t)spme->N;
+  for (int nn = 0; nn < nrhs; ++nn) {
+    const double *vin = vec_in + (size_t)nn * ldin;
+    double *vout = vec_out + (size_t)nn * ldout;
+    if (hipMemcpyAsync(spme->h_in, vin, n3 * sizeof(double), hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        (beta != 0.0 && hipMemcpyAsync(spme->h_out, vout, n3 * sizeof(double),
+                                       hipMemcpyHostToDevice, st) != hipSuccess)) {
+      fail(-2, "host-to-device copy failed");
+      die("compute_spme");
+    }
+    if (apply(spme, 1, alpha, spme->h_in, (long)n3, beta, spme->h_out, (long)n3, SPME_RECIP,
+              st))
+      die("compute_spme");
+    if (hipMemcpyAsync(vout, spme->h_out, n3 * sizeof(double), hipMemcpyDeviceToHost, st) !=
+            hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      fail(-2, "device-to-host copy failed");
+      die("compute_spme");
+    }
+  }
+}
+
+}  // extern "C"

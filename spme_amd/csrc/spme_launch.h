@@ -1,0 +1,63 @@
+// Host-side launch interface of the SPME kernels (spme_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace spme {
+
+template <typename T> struct cpx;
+
+struct WTable {
+  double w[64];  // W[x*P + y], P <= 8
+};
+
+template <typename T> struct SpreadArgs {
+  const int *boff;
+  const int *pbase;
+  const T *pw;
+  const T *fs;
+  int C, CC, P;
+  T *grid;
+  long comp_stride;
+  int RS, K, ZC;
+};
+
+template <typename T> struct InterpArgs {
+  const int *pbase;
+  const T *pw;
+  const int *perm;
+  const T *grid;
+  long comp_stride;
+  int RS, K, N, C, P;
+  T alpha, beta;
+  T *out;
+  long ldout;
+};
+
+template <typename T> struct FftArgs {
+  T *grid;
+  long comp_stride;
+  int RS, K, ncomp;
+  const T *tab;
+  T dk;
+  const cpx<T> *tw;
+};
+
+void launch_pbuild(const double *pos, int N, double L, int K, int P, const WTable &W, int TX,
+                   int TY, int *base_tmp, double *w_tmp, int *key_tmp, int *counts, int *err,
+                   hipStream_t st);
+void launch_scan(const int *in, int *out, int n, hipStream_t st);
+void launch_pscatter(const int *key, const int *boff, int *cursor, int *perm, int N,
+                     hipStream_t st);
+void launch_bucket_sort(const int *boff, int *perm, int nkeys, hipStream_t st);
+template <typename T>
+void launch_pgather(const int *perm, const int *base_tmp, const double *w_tmp, int *pbase,
+                    T *pw, int N, int P3, hipStream_t st);
+template <typename T>
+void launch_permute_in(const int *perm, const T *in, long ldin, T *fs, int N, int C,
+                       hipStream_t st);
+template <typename T> int launch_spread(const SpreadArgs<T> &a, hipStream_t st);
+template <typename T> int launch_interp(const InterpArgs<T> &a, hipStream_t st);
+template <typename T> int launch_fft_influence(const FftArgs<T> &a, hipStream_t st);
+bool fft_size_supported(int K);
+
+}  // namespace spme

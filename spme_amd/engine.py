@@ -1,0 +1,168 @@
+"""Python mirror of the engine interface.
+
+`Engine` wraps the extension ABI (include/spme_ext.h): device-resident
+inputs, FP64 or FP32 mesh, batches of right-hand sides, full or partial M.
+`DropIn` wraps the reference's own surface (include/spme.h:
+create_spme_engine / compute_spme / destroy_spme_engine, old-version/spme.h:
+85-97) with host numpy arrays, the way the reference's driver calls it
+(old-version/main.c:215-250).
+
+torch is used only to hold device memory and to name the current stream.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import lib as _L
+from .lib import (SPME_FP32, SPME_FP64, SPME_FULL, SPME_REAL, SPME_RECIP,
+                  SPME_SELF, SpmeError, SpmeInfo, SpmeParams, check, load)
+
+__all__ = ["Engine", "DropIn", "cutoff2xi", "SPME_FP32", "SPME_FP64",
+           "SPME_FULL", "SPME_REAL", "SPME_RECIP", "SPME_SELF", "SpmeError"]
+
+
+def cutoff2xi(rmax: float, ewald_eps: float, box_size: float) -> float:
+    """old-version/spme.h:85-86 (declared only in the reference)."""
+    xi = ctypes.c_double()
+    load().cutoff2xi(float(rmax), float(ewald_eps), float(box_size),
+                     ctypes.byref(xi))
+    return xi.value
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise SpmeError("no GPU visible: the SPME engine runs only on the GPU")
+    return torch
+
+
+class Engine:
+    """One SPME engine (positions, P data, mesh) on the current GPU."""
+
+    def __init__(self, xi, dim, porder, np_, box_size, rc=0.0,
+                 precision=SPME_FP64, max_rhs=1):
+        self._lib = load()
+        prm = SpmeParams(float(xi), int(dim), int(porder), int(np_),
+                         float(box_size), float(rc), int(precision),
+                         int(max_rhs))
+        h = ctypes.c_void_p()
+        check(self._lib.spme_engine_create(ctypes.byref(prm), ctypes.byref(h)),
+              "spme_engine_create")
+        self._h = h
+        self.N = int(np_)
+        self.precision = int(precision)
+        self.max_rhs = int(max_rhs)
+
+    @property
+    def dtype(self):
+        torch = _torch()
+        return torch.float32 if self.precision == SPME_FP32 else torch.float64
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.spme_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self) -> SpmeInfo:
+        inf = SpmeInfo()
+        check(self._lib.spme_engine_info(self._h, ctypes.byref(inf)), "info")
+        return inf
+
+    def set_positions(self, pos, stream=None):
+        """pos: torch CUDA tensor [N, 3] float64, or a host numpy array."""
+        if isinstance(pos, np.ndarray):
+            a = np.ascontiguousarray(pos, dtype=np.float64)
+            check(self._lib.spme_engine_set_positions(
+                self._h, a.ctypes.data_as(ctypes.c_void_p), 0, None),
+                "set_positions")
+            return
+        torch = _torch()
+        assert pos.dtype == torch.float64 and pos.is_cuda and pos.is_contiguous()
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(self._lib.spme_engine_set_positions(
+            self._h, ctypes.c_void_p(pos.data_ptr()), 1, ctypes.c_void_p(s)),
+            "set_positions")
+
+    def apply(self, f, u=None, alpha=1.0, beta=0.0, parts=SPME_RECIP,
+              stream=None):
+        """u <- beta*u + alpha*M f for f [nrhs, 3N] (torch CUDA tensor of the
+        engine's precision, last dim contiguous).  Returns u."""
+        torch = _torch()
+        if f.dim() == 1:
+            f = f.view(1, -1)
+        if u is None:
+            u = torch.zeros_like(f)
+            beta = 0.0
+        nrhs = f.shape[0]
+        assert f.dtype == self.dtype and u.dtype == self.dtype
+        assert f.stride(1) == 1 and u.stride(1) == 1
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(self._lib.spme_engine_apply(
+            self._h, int(nrhs), float(alpha), ctypes.c_void_p(f.data_ptr()),
+            int(f.stride(0)), float(beta), ctypes.c_void_p(u.data_ptr()),
+            int(u.stride(0)), int(parts), ctypes.c_void_p(s)), "apply")
+        return u
+
+    def debug_stage(self, stage, f):
+        """Mesh after spreading (stage 1) or after FFT, influence and inverse
+        FFT (stage 2), as numpy [3*nrhs, K, K, RS]."""
+        inf = self.info()
+        f = f.view(1, -1) if f.dim() == 1 else f
+        nrhs = f.shape[0]
+        dt = np.float32 if self.precision == SPME_FP32 else np.float64
+        out = np.empty(3 * nrhs * inf.comp_stride, dtype=dt)
+        check(self._lib.spme_engine_debug_stage(
+            self._h, int(stage), int(nrhs), ctypes.c_void_p(f.data_ptr()),
+            int(f.stride(0)), out.ctypes.data_as(ctypes.c_void_p)), "debug_stage")
+        return out.reshape(3 * nrhs, inf.K, inf.K, inf.RS)
+
+
+class DropIn:
+    """The reference's C surface, called with host arrays."""
+
+    def __init__(self, xi, dim, porder, np_, box_size, nthreads=1):
+        self._lib = load()
+        h = ctypes.c_void_p()
+        self._lib.create_spme_engine(float(xi), int(dim), int(porder),
+                                     int(np_), float(box_size), int(nthreads),
+                                     ctypes.byref(h))
+        self._h = h
+        self.N = int(np_)
+
+    def compute(self, pos, nrhs, alpha, vec_in, ldin, beta, vec_out, ldout):
+        p = None
+        if pos is not None:
+            pos = np.ascontiguousarray(pos, dtype=np.float64)
+            p = pos.ctypes.data_as(ctypes.c_void_p)
+        assert vec_in.dtype == np.float64 and vec_out.dtype == np.float64
+        assert vec_in.flags.c_contiguous and vec_out.flags.c_contiguous
+        self._lib.compute_spme(self._h, p, int(nrhs), float(alpha),
+                               vec_in.ctypes.data_as(ctypes.c_void_p), int(ldin),
+                               float(beta),
+                               vec_out.ctypes.data_as(ctypes.c_void_p),
+                               int(ldout))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.destroy_spme_engine(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

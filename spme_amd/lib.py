@@ -1,0 +1,81 @@
+"""ctypes binding of libspme_hip.so (the C ABI of include/spme.h and
+include/spme_ext.h).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly
+if the shared library has not been built (spme_amd/build.py).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspme_hip.so")
+
+SPME_FP64, SPME_FP32 = 0, 1
+SPME_RECIP, SPME_REAL, SPME_SELF, SPME_FULL = 1, 2, 4, 7
+
+# Every symbol the public headers declare.
+EXPORTED = [
+    "cutoff2xi", "create_spme_engine", "destroy_spme_engine", "compute_spme",
+    "spme_last_error", "spme_engine_create", "spme_engine_destroy",
+    "spme_engine_set_positions", "spme_engine_apply", "spme_engine_info",
+    "spme_engine_debug_stage",
+]
+
+
+class SpmeParams(ctypes.Structure):
+    _fields_ = [("xi", ctypes.c_double), ("dim", ctypes.c_int),
+                ("porder", ctypes.c_int), ("np", ctypes.c_int),
+                ("box_size", ctypes.c_double), ("rc", ctypes.c_double),
+                ("precision", ctypes.c_int), ("max_rhs", ctypes.c_int)]
+
+
+class SpmeInfo(ctypes.Structure):
+    _fields_ = [("K", ctypes.c_int), ("P", ctypes.c_int), ("N", ctypes.c_int),
+                ("precision", ctypes.c_int), ("max_rhs", ctypes.c_int),
+                ("RS", ctypes.c_int), ("comp_stride", ctypes.c_long),
+                ("L", ctypes.c_double), ("xi", ctypes.c_double),
+                ("rc", ctypes.c_double), ("mesh_bytes", ctypes.c_long),
+                ("stream", ctypes.c_void_p)]
+
+
+class SpmeError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SpmeError(f"{LIB_PATH} is missing: build it with "
+                        "`python -m spme_amd.build` (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, d, i, l = ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_long
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.cutoff2xi.argtypes = [d, d, d, dp]
+    L.create_spme_engine.argtypes = [d, i, i, i, d, i, ctypes.POINTER(vp)]
+    L.destroy_spme_engine.argtypes = [vp]
+    L.compute_spme.argtypes = [vp, vp, i, d, vp, i, d, vp, i]
+    L.spme_last_error.restype = ctypes.c_char_p
+    L.spme_engine_create.argtypes = [ctypes.POINTER(SpmeParams), ctypes.POINTER(vp)]
+    L.spme_engine_destroy.argtypes = [vp]
+    L.spme_engine_set_positions.argtypes = [vp, vp, i, vp]
+    L.spme_engine_apply.argtypes = [vp, i, d, vp, l, d, vp, l, i, vp]
+    L.spme_engine_info.argtypes = [vp, ctypes.POINTER(SpmeInfo)]
+    L.spme_engine_debug_stage.argtypes = [vp, i, i, vp, l, vp]
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return load().spme_last_error().decode()
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise SpmeError(f"{what}: {last_error()} (code {rc})")

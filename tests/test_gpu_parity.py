@@ -1,0 +1,215 @@
+"""GPU parity of the HIP engine against the CPU oracle (restated reference
+SPME, old-version/spme.c + spme_kernel.c) through the C ABI.
+
+FP64 bar: relative max-norm difference <= 1e-12 (same algorithm, different
+summation order and fused multiply-adds).  FP32 bar: <= 2e-5 relative
+(FP32 mesh, spreading, FFT and interpolation; FP64 tables).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from spme_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 2e-5
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _rel(a, b):
+    return float(np.abs(np.asarray(a) - np.asarray(b)).max() / np.abs(np.asarray(b)).max())
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    if not t.cuda.is_available():
+        pytest.skip("no GPU")
+    return t
+
+
+@pytest.fixture(scope="module")
+def E():
+    from spme_amd import engine
+    return engine
+
+
+def _system(seed, n, phi):
+    s = W.make_system(seed, n, phi)
+    return s.pos, s.L
+
+
+def test_dropin_matches_oracle_c1(oracle, E, torch):
+    """Config 1 through the reference's own surface, with the reference
+    driver's alpha=2.1, beta=2.2 (old-version/main.c:224-245)."""
+    _say("case: drop-in compute_spme, N=1000 phi=0.1 K=32 p=6")
+    pos, L = _system(1001, 1000, 0.1)
+    xi = E.cutoff2xi(5.0, 1e-6, L)
+    f = W.forces(1001, 1000, 2)
+    v0 = W.forces(1002, 1000, 2)
+    ref = oracle.compute_spme(xi, 32, 6, L, pos, f, alpha=2.1, beta=2.2, u=v0)
+    d = E.DropIn(xi, 32, 6, 1000, L)
+    v = v0.copy()
+    d.compute(pos, 2, 2.1, f, 3000, 2.2, v, 3000)
+    assert _rel(v, ref) < FP64_TOL
+    # pos == NULL reuses the P data
+    v2 = v0.copy()
+    d.compute(None, 2, 2.1, f, 3000, 2.2, v2, 3000)
+    np.testing.assert_array_equal(v2, v)
+    d.close()
+
+
+@pytest.mark.parametrize("p", [4, 6, 8])
+def test_spread_and_fft_stages(oracle, E, torch, p):
+    _say(f"case: stages p={p} K=32 N=300")
+    N, K = 300, 32
+    pos, L = _system(77 + p, N, 0.15)
+    xi = 0.6
+    f = W.forces(77 + p, N)[0]
+    eng = E.Engine(xi, K, p, N, L, precision=E.SPME_FP64)
+    eng.set_positions(pos)
+    ft = torch.tensor(f, device="cuda").view(1, -1)
+    m1 = eng.debug_stage(1, ft)[:, :, :, :K]
+    P_, ind, _, _ = oracle.compute_P(K, p, pos, L)
+    g = oracle.spread(K, p, P_, ind, f)
+    assert _rel(m1, g) < 1e-13
+    m2 = eng.debug_stage(2, ft)[:, :, :, :K]
+    lm2 = oracle.compute_lm2(xi, K, p, L)
+    spec = np.stack([oracle.fft3d_r2c(g[c]) for c in range(3)])
+    spec = oracle.apply_influence(K, L, lm2, spec)
+    back = np.stack([oracle.fft3d_c2r(spec[c]) for c in range(3)]) / L ** 3
+    assert _rel(m2, back) < FP64_TOL
+    eng.close()
+
+
+@pytest.mark.parametrize("K,p", [(32, 4), (48, 6), (64, 8)])
+def test_recip_fp64_and_fp32(oracle, E, torch, K, p):
+    _say(f"case: recip K={K} p={p} N=1500 fp64+fp32")
+    N = 1500
+    pos, L = _system(5 + K, N, 0.2)
+    xi = 0.55
+    f = W.forces(5 + K, N, 1)
+    ref = oracle.compute_spme(xi, K, p, L, pos, f)
+    for prec, tol, dt in [(E.SPME_FP64, FP64_TOL, torch.float64),
+                          (E.SPME_FP32, FP32_TOL, torch.float32)]:
+        eng = E.Engine(xi, K, p, N, L, precision=prec)
+        eng.set_positions(torch.tensor(pos, device="cuda"))
+        u = eng.apply(torch.tensor(f, device="cuda", dtype=dt))
+        torch.cuda.synchronize()
+        assert _rel(u.double().cpu().numpy(), ref) < tol, prec
+        eng.close()
+
+
+def test_batched_rhs_equal_single(oracle, E, torch):
+    _say("case: batched R=16 fp32 and R=4 fp64, K=48 p=8 N=2000")
+    N, K, p = 2000, 48, 8
+    pos, L = _system(91, N, 0.2)
+    xi = 0.5
+    f = W.forces(91, N, 16)
+    ref = oracle.compute_spme(xi, K, p, L, pos, f)
+    eng = E.Engine(xi, K, p, N, L, precision=E.SPME_FP32, max_rhs=16)
+    eng.set_positions(pos)
+    u = eng.apply(torch.tensor(f, device="cuda", dtype=torch.float32))
+    got = u.double().cpu().numpy()
+    for r in range(16):
+        assert _rel(got[r], ref[r]) < FP32_TOL, r
+    eng.close()
+    eng = E.Engine(xi, K, p, N, L, precision=E.SPME_FP64, max_rhs=4)
+    eng.set_positions(pos)
+    u = eng.apply(torch.tensor(f[:4], device="cuda"))
+    single = E.Engine(xi, K, p, N, L, precision=E.SPME_FP64, max_rhs=1)
+    single.set_positions(pos)
+    for r in range(4):
+        assert _rel(u[r].cpu().numpy(), ref[r]) < FP64_TOL
+        s = single.apply(torch.tensor(f[r], device="cuda"))
+        # same kernels and summation order: batching changes nothing
+        np.testing.assert_array_equal(s[0].cpu().numpy(), u[r].cpu().numpy())
+    eng.close()
+    single.close()
+
+
+def test_strided_alpha_beta(oracle, E, torch):
+    _say("case: ldin/ldout > 3N, alpha/beta")
+    N, K, p = 500, 32, 6
+    pos, L = _system(13, N, 0.1)
+    xi = 0.5
+    f = W.forces(13, N, 3)
+    u0 = W.forces(14, N, 3)
+    ref = oracle.compute_spme(xi, K, p, L, pos, f, alpha=-0.7, beta=1.9, u=u0)
+    eng = E.Engine(xi, K, p, N, L, max_rhs=3)
+    eng.set_positions(pos)
+    fin = torch.zeros(3, 3 * N + 16, dtype=torch.float64, device="cuda")
+    fin[:, :3 * N] = torch.tensor(f)
+    uo = torch.full((3, 3 * N + 40), 7.0, dtype=torch.float64, device="cuda")
+    uo[:, :3 * N] = torch.tensor(u0)
+    eng.apply(fin[:, :3 * N], uo[:, :3 * N], alpha=-0.7, beta=1.9)
+    got = uo.cpu().numpy()
+    assert _rel(got[:, :3 * N], ref) < FP64_TOL
+    assert np.all(got[:, 3 * N:] == 7.0)   # padding untouched
+    eng.close()
+
+
+def test_full_mobility_vs_oracle_and_direct(oracle, E, torch):
+    _say("case: full M.f fp64/fp32 vs oracle spme_full and direct Ewald")
+    N = 1000
+    pos, L = _system(1001, N, 0.1)
+    rc = 5.0
+    xi = E.cutoff2xi(rc, 1e-9, L)
+    f = W.forces(1001, N, 2)
+    ref = oracle.spme_full(xi, 64, 8, rc, L, pos, f)
+    for prec, tol, dt in [(E.SPME_FP64, FP64_TOL, torch.float64),
+                          (E.SPME_FP32, FP32_TOL, torch.float32)]:
+        eng = E.Engine(xi, 64, 8, N, L, rc=rc, precision=prec, max_rhs=2)
+        eng.set_positions(pos)
+        u = eng.apply(torch.tensor(f, device="cuda", dtype=dt), parts=E.SPME_FULL)
+        got = u.double().cpu().numpy()
+        assert _rel(got, ref) < tol, prec
+        eng.close()
+    rows = W.rows(3, N, 200)
+    direct = oracle.ewald_direct(pos, L, f[0], rows=rows)
+    mine = got[0].reshape(N, 3)[rows].reshape(-1)
+    err = np.linalg.norm(mine - direct) / np.linalg.norm(direct)
+    assert err < 1e-4, err
+
+
+def test_deterministic(E, torch):
+    _say("case: run twice bitwise, fp32 K=48 p=6 R=4")
+    N, K, p = 3000, 48, 6
+    pos, L = _system(55, N, 0.25)
+    f = torch.tensor(W.forces(55, N, 4), device="cuda", dtype=torch.float32)
+    outs = []
+    for _ in range(2):
+        eng = E.Engine(0.6, K, p, N, L, rc=4.0, precision=E.SPME_FP32, max_rhs=4)
+        eng.set_positions(pos)
+        outs.append(eng.apply(f, parts=E.SPME_FULL).cpu().numpy())
+        eng.close()
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
+def test_errors_are_reported(E, torch):
+    _say("case: argument errors")
+    with pytest.raises(E.SpmeError, match="porder"):
+        E.Engine(0.5, 32, 5, 100, 10.0)
+    with pytest.raises(E.SpmeError, match="dim"):
+        E.Engine(0.5, 40, 4, 100, 10.0)
+    with pytest.raises(E.SpmeError, match="4 \\* porder"):
+        E.Engine(0.5, 32, 8, 100, 10.0)
+    eng = E.Engine(0.5, 32, 6, 10, 10.0, max_rhs=2)
+    f = torch.zeros(3, 30, dtype=torch.float64, device="cuda")
+    with pytest.raises(E.SpmeError, match="positions"):
+        eng.apply(f[:1])
+    bad = np.zeros((10, 3))
+    bad[3, 1] = np.nan
+    with pytest.raises(E.SpmeError, match="finite"):
+        eng.set_positions(bad)
+    eng.set_positions(np.full((10, 3), 1.0))
+    with pytest.raises(E.SpmeError, match="nrhs"):
+        eng.apply(f)
+    eng.close()
