@@ -44,16 +44,36 @@ int spme_engine_create(const spme_params_t *prm, spme_t **out);
 int spme_engine_destroy(spme_t *spme);
 
 /* Build the interpolation (P) data from positions (N x 3 FP64).  pos is a
- * device pointer if on_device != 0, else host memory.  Asynchronous on
- * stream (NULL: the engine's own stream) for device input. */
+ * device pointer if on_device != 0, else host memory.  stream is a
+ * hipStream_t (NULL: the default stream); the call waits for the build. */
 int spme_engine_set_positions(spme_t *spme, const double *pos, int on_device, void *stream);
 
 /* out[r*ldout + 3i + c] <- beta*out + alpha*(M f_r)[3i + c], r < nrhs,
  * for the parts in `parts`.  in/out are device pointers of the engine's
  * precision (float for SPME_FP32, double for SPME_FP64).  Asynchronous on
- * stream (NULL: the engine's own stream). */
+ * stream, a hipStream_t (NULL: the default stream). */
 int spme_engine_apply(spme_t *spme, int nrhs, double alpha, const void *in, long ldin,
                       double beta, void *out, long ldout, int parts, void *stream);
+
+/* Per-kernel timing.  With timing on, every apply records HIP events on its
+ * stream after each kernel launch; get_timing waits for them and returns,
+ * per stage, the summed milliseconds and the number of launches since
+ * timing was (re)enabled.  Stages, in launch order: */
+enum {
+  SPME_ST_PERMUTE = 0,   /* forces into sorted order (reciprocal part) */
+  SPME_ST_SPREAD,        /* B-spline spreading onto the mesh */
+  SPME_ST_FFT_X_FWD,     /* r2c along x */
+  SPME_ST_FFT_Y_FWD,     /* complex FFT along y */
+  SPME_ST_FFT_Z_INFL,    /* FFT along z, influence, inverse FFT along z */
+  SPME_ST_FFT_Y_INV,     /* inverse FFT along y */
+  SPME_ST_FFT_X_INV,     /* c2r along x */
+  SPME_ST_INTERP,        /* interpolation back to particles */
+  SPME_ST_PERMUTE_REAL,  /* forces into cell order (real-space part) */
+  SPME_ST_REAL,          /* real-space pairs, overlap and self */
+  SPME_NSTAGES
+};
+int spme_engine_set_timing(spme_t *spme, int enable);
+int spme_engine_get_timing(spme_t *spme, double *ms_total, int *launches);
 
 /* Layout and sizes, for tests and benchmarks. */
 typedef struct spme_info {

@@ -119,6 +119,11 @@ double psi(double k, double xi) {
 
 // ---------------------------------------------------------------------------
 
+struct StageEvents {
+  hipEvent_t start;
+  std::vector<std::pair<int, hipEvent_t>> marks;
+};
+
 struct _spme_t {
   spme_params_t prm;
   int K, P, N, prec, max_rhs, Kh, Khp, RS;
@@ -145,6 +150,13 @@ struct _spme_t {
   void *r_fs;
   // drop-in staging (FP64 host API)
   double *h_in, *h_out;
+  // per-kernel timing: when enabled, events are recorded on the apply
+  // stream after every launch; a stage's time is the gap to the event before
+  int timing;
+  std::vector<StageEvents> pending;
+  std::vector<hipEvent_t> pool;
+  double stage_ms[SPME_NSTAGES];
+  int stage_n[SPME_NSTAGES];
 };
 
 namespace {
@@ -209,13 +221,17 @@ void free_all(_spme_t *e) {
                   e->r_fs, e->h_in, e->h_out};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  for (auto &r : e->pending) {
+    e->pool.push_back(r.start);
+    for (auto &m : r.marks) e->pool.push_back(m.second);
+  }
+  for (hipEvent_t ev : e->pool) (void)hipEventDestroy(ev);
   if (e->stream) (void)hipStreamDestroy(e->stream);
 }
 
 int engine_create(const spme_params_t *prm, _spme_t **out) {
   if (int rc = validate(prm)) return rc;
-  _spme_t *e = new _spme_t();
-  std::memset(e, 0, sizeof(*e));
+  _spme_t *e = new _spme_t();  // value-initialised: every pointer starts null
   e->prm = *prm;
   e->K = prm->dim;
   e->P = prm->porder;
@@ -321,6 +337,35 @@ int set_positions(_spme_t *e, const double *pos, int on_device, hipStream_t st) 
   return 0;
 }
 
+hipEvent_t take_event(_spme_t *e) {
+  if (!e->pool.empty()) {
+    hipEvent_t ev = e->pool.back();
+    e->pool.pop_back();
+    return ev;
+  }
+  hipEvent_t ev = nullptr;
+  (void)hipEventCreate(&ev);
+  return ev;
+}
+
+int collect_timing(_spme_t *e) {
+  for (auto &r : e->pending) {
+    hipEvent_t prev = r.start;
+    for (auto &m : r.marks) {
+      HIP_TRY(hipEventSynchronize(m.second));
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, prev, m.second));
+      e->stage_ms[m.first] += ms;
+      e->stage_n[m.first] += 1;
+      prev = m.second;
+    }
+    e->pool.push_back(r.start);
+    for (auto &m : r.marks) e->pool.push_back(m.second);
+  }
+  e->pending.clear();
+  return 0;
+}
+
 int pick_cc(int C, int P) {
   if (P < 8 && C % 12 == 0) return 12;
   return C % 6 == 0 ? 6 : 3;
@@ -330,41 +375,64 @@ template <typename T>
 int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double beta, T *out,
             long ldout, int parts, hipStream_t st, int stop_after) {
   const int N = e->N, C = 3 * nrhs;
+  if ((parts & SPME_SELF) && !(parts & SPME_REAL))
+    return fail(-1, "SPME_SELF without SPME_REAL is not supported");
+  if ((parts & SPME_REAL) && e->nc < 3)
+    return fail(-1, "real-space part needs rc > 0 and box_size >= 3*max(rc, 2)");
+  StageEvents rec;
+  const bool timed = e->timing && stop_after == 0;
+  if (timed) {
+    rec.start = take_event(e);
+    HIP_TRY(hipEventRecord(rec.start, st));
+  }
+  auto mark = [&](int stage) -> int {
+    if (!timed) return 0;
+    hipEvent_t ev = take_event(e);
+    HIP_TRY(hipEventRecord(ev, st));
+    rec.marks.push_back({stage, ev});
+    return 0;
+  };
   if (parts & SPME_RECIP) {
     launch_permute_in<T>(e->perm, in, ldin, (T *)e->fs, N, C, st);
-    SpreadArgs<T> sa{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, C, pick_cc(C, e->P), e->P,
-                     (T *)e->grid, e->comp_stride, e->RS, e->K,
+    mark(SPME_ST_PERMUTE);
+    SpreadArgs<T> sa{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, C, pick_cc(C, e->P),
+                     e->P, (T *)e->grid, e->comp_stride, e->RS, e->K,
                      e->K >= 64 ? e->K / 4 : e->K / 2};
     if (launch_spread<T>(sa, st)) return fail(-1, "spread: unsupported configuration");
+    mark(SPME_ST_SPREAD);
     if (stop_after == 1) return 0;
     FftArgs<T> fa{(T *)e->grid, e->comp_stride, e->RS, e->K, C, (const T *)e->tab,
                   T(2.0 * M_PI / e->prm.box_size), (const cpx<T> *)e->tw};
-    if (launch_fft_influence<T>(fa, st)) return fail(-1, "fft: unsupported size");
+    hipEvent_t fev[5];
+    if (timed)
+      for (int i = 0; i < 5; ++i) fev[i] = take_event(e);
+    if (launch_fft_influence<T>(fa, st, timed ? fev : nullptr))
+      return fail(-1, "fft: unsupported size");
+    if (timed)
+      for (int i = 0; i < 5; ++i) rec.marks.push_back({SPME_ST_FFT_X_FWD + i, fev[i]});
     if (stop_after == 2) return 0;
     InterpArgs<T> ia{e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid, e->comp_stride,
                      e->RS, e->K, N, C, e->P, T(alpha), T(beta), out, ldout};
     if (launch_interp<T>(ia, st)) return fail(-1, "interpolate: unsupported configuration");
+    mark(SPME_ST_INTERP);
   }
-  if (parts & (SPME_REAL | SPME_SELF)) {
-    if (e->nc < 3)
-      return fail(-1, "real-space part needs box_size >= 3*max(rc, 2) (cell list)");
+  if (parts & SPME_REAL) {
     launch_permute_in<T>(e->r_perm, in, ldin, (T *)e->r_fs, N, C, st);
+    mark(SPME_ST_PERMUTE_REAL);
     const double xi = e->prm.xi;
     const double self = (parts & SPME_SELF)
                             ? 1.0 - 6.0 * xi / std::sqrt(M_PI) +
                                   40.0 * xi * xi * xi / (3.0 * std::sqrt(M_PI))
                             : 0.0;
-    const double rc2 = (parts & SPME_REAL) ? e->prm.rc * e->prm.rc : 0.0;
+    const double rc2 = e->prm.rc * e->prm.rc;
     RealArgs<T> ra{e->r_pos, e->r_perm, e->r_off, e->nc, e->prm.box_size, xi, rc2, self,
                    (const T *)e->r_fs, N, C, alpha, beta, (parts & SPME_RECIP) ? 1 : 0,
                    out, ldout};
-    if (!(parts & SPME_REAL)) {
-      // self only: no pairs at all (rc2 = 0 still adds overlap; disable it)
-      return fail(-1, "SPME_SELF without SPME_REAL is not supported");
-    }
     if (launch_real<T>(ra, st)) return fail(-1, "real space: unsupported batch width");
+    mark(SPME_ST_REAL);
   }
   HIP_TRY(hipGetLastError());
+  if (timed) e->pending.push_back(std::move(rec));
   return 0;
 }
 
@@ -413,15 +481,34 @@ int spme_engine_destroy(spme_t *spme) {
 
 int spme_engine_set_positions(spme_t *spme, const double *pos, int on_device, void *stream) {
   if (!spme) return fail(-1, "null engine");
-  hipStream_t st = stream ? (hipStream_t)stream : spme->stream;
-  return set_positions(spme, pos, on_device, st);
+  return set_positions(spme, pos, on_device, (hipStream_t)stream);
 }
 
 int spme_engine_apply(spme_t *spme, int nrhs, double alpha, const void *in, long ldin,
                       double beta, void *out, long ldout, int parts, void *stream) {
   if (!spme) return fail(-1, "null engine");
-  hipStream_t st = stream ? (hipStream_t)stream : spme->stream;
-  return apply(spme, nrhs, alpha, in, ldin, beta, out, ldout, parts, st);
+  return apply(spme, nrhs, alpha, in, ldin, beta, out, ldout, parts, (hipStream_t)stream);
+}
+
+int spme_engine_set_timing(spme_t *e, int enable) {
+  if (!e) return fail(-1, "null engine");
+  if (int rc = collect_timing(e)) return rc;
+  e->timing = enable ? 1 : 0;
+  for (int i = 0; i < SPME_NSTAGES; ++i) {
+    e->stage_ms[i] = 0.0;
+    e->stage_n[i] = 0;
+  }
+  return 0;
+}
+
+int spme_engine_get_timing(spme_t *e, double *ms_total, int *launches) {
+  if (!e || !ms_total || !launches) return fail(-1, "null argument");
+  if (int rc = collect_timing(e)) return rc;
+  for (int i = 0; i < SPME_NSTAGES; ++i) {
+    ms_total[i] = e->stage_ms[i];
+    launches[i] = e->stage_n[i];
+  }
+  return 0;
 }
 
 int spme_engine_info(const spme_t *e, spme_info_t *info) {

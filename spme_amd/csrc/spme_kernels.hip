@@ -566,7 +566,7 @@ template <typename T> struct FftShape {
 };
 
 template <typename T, int K>
-static void fft3d_k(const FftArgs<T> &a, hipStream_t st) {
+static void fft3d_k(const FftArgs<T> &a, hipStream_t st, hipEvent_t *ev) {
   using S = FftShape<T>;
   constexpr int NT = 256;
   const long rows = (long)a.ncomp * K * K;
@@ -574,30 +574,38 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st) {
   const int Khp = a.RS / 2;
   const int nxc = (K / 2 + 1 + S::XW - 1) / S::XW;
   const int nxcz = (K / 2 + 1 + S::XWZ - 1) / S::XWZ;
+  auto mark = [&](int i) {
+    if (ev) (void)hipEventRecord(ev[i], st);
+  };
   hipLaunchKernelGGL((k_fft_x_r2c<T, K, S::NLX, NT>), dim3(grid1(npairs, S::NLX)), dim3(NT), 0,
                      st, a.grid, a.RS, npairs, a.tw);
+  mark(0);
   hipLaunchKernelGGL((k_fft_y<T, K, S::XW, -1, NT>), dim3(nxc * K * a.ncomp), dim3(NT), 0, st,
                      a.grid, a.comp_stride, Khp, nxc, a.tw);
+  mark(1);
   hipLaunchKernelGGL((k_fft_z_infl<T, K, S::XWZ, NT>), dim3(nxcz * K, a.ncomp / 3), dim3(NT), 0,
                      st, a.grid, a.comp_stride, Khp, nxcz, a.tab, a.dk, a.tw);
+  mark(2);
   hipLaunchKernelGGL((k_fft_y<T, K, S::XW, +1, NT>), dim3(nxc * K * a.ncomp), dim3(NT), 0, st,
                      a.grid, a.comp_stride, Khp, nxc, a.tw);
+  mark(3);
   hipLaunchKernelGGL((k_fft_x_c2r<T, K, S::NLX, NT>), dim3(grid1(npairs, S::NLX)), dim3(NT), 0,
                      st, a.grid, a.RS, npairs, a.tw);
+  mark(4);
 }
 
 template <typename T>
-int launch_fft_influence(const FftArgs<T> &a, hipStream_t st) {
+int launch_fft_influence(const FftArgs<T> &a, hipStream_t st, hipEvent_t *ev) {
   switch (a.K) {
-    case 16: fft3d_k<T, 16>(a, st); return 0;
-    case 24: fft3d_k<T, 24>(a, st); return 0;
-    case 32: fft3d_k<T, 32>(a, st); return 0;
-    case 48: fft3d_k<T, 48>(a, st); return 0;
-    case 64: fft3d_k<T, 64>(a, st); return 0;
-    case 96: fft3d_k<T, 96>(a, st); return 0;
-    case 128: fft3d_k<T, 128>(a, st); return 0;
-    case 192: fft3d_k<T, 192>(a, st); return 0;
-    case 256: fft3d_k<T, 256>(a, st); return 0;
+    case 16: fft3d_k<T, 16>(a, st, ev); return 0;
+    case 24: fft3d_k<T, 24>(a, st, ev); return 0;
+    case 32: fft3d_k<T, 32>(a, st, ev); return 0;
+    case 48: fft3d_k<T, 48>(a, st, ev); return 0;
+    case 64: fft3d_k<T, 64>(a, st, ev); return 0;
+    case 96: fft3d_k<T, 96>(a, st, ev); return 0;
+    case 128: fft3d_k<T, 128>(a, st, ev); return 0;
+    case 192: fft3d_k<T, 192>(a, st, ev); return 0;
+    case 256: fft3d_k<T, 256>(a, st, ev); return 0;
   }
   return -1;
 }
@@ -619,7 +627,7 @@ template int launch_spread<float>(const SpreadArgs<float> &, hipStream_t);
 template int launch_spread<double>(const SpreadArgs<double> &, hipStream_t);
 template int launch_interp<float>(const InterpArgs<float> &, hipStream_t);
 template int launch_interp<double>(const InterpArgs<double> &, hipStream_t);
-template int launch_fft_influence<float>(const FftArgs<float> &, hipStream_t);
-template int launch_fft_influence<double>(const FftArgs<double> &, hipStream_t);
+template int launch_fft_influence<float>(const FftArgs<float> &, hipStream_t, hipEvent_t *);
+template int launch_fft_influence<double>(const FftArgs<double> &, hipStream_t, hipEvent_t *);
 
 }  // namespace spme

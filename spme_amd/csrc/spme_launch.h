@@ -57,7 +57,9 @@ void launch_permute_in(const int *perm, const T *in, long ldin, T *fs, int N, in
                        hipStream_t st);
 template <typename T> int launch_spread(const SpreadArgs<T> &a, hipStream_t st);
 template <typename T> int launch_interp(const InterpArgs<T> &a, hipStream_t st);
-template <typename T> int launch_fft_influence(const FftArgs<T> &a, hipStream_t st);
+// ev (optional): 5 events recorded after each of the 5 FFT-pass launches.
+template <typename T>
+int launch_fft_influence(const FftArgs<T> &a, hipStream_t st, hipEvent_t *ev = nullptr);
 bool fft_size_supported(int K);
 
 }  // namespace spme

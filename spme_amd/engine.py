@@ -117,6 +117,18 @@ class Engine:
             int(u.stride(0)), int(parts), ctypes.c_void_p(s)), "apply")
         return u
 
+    def set_timing(self, enable=True):
+        check(self._lib.spme_engine_set_timing(self._h, int(bool(enable))),
+              "set_timing")
+
+    def timing(self):
+        """{stage: (total ms, launches)} since timing was enabled."""
+        n = len(_L.STAGES)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_int * n)()
+        check(self._lib.spme_engine_get_timing(self._h, ms, cnt), "get_timing")
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(_L.STAGES)}
+
     def debug_stage(self, stage, f):
         """Mesh after spreading (stage 1) or after FFT, influence and inverse
         FFT (stage 2), as numpy [3*nrhs, K, K, RS]."""

@@ -20,8 +20,11 @@ EXPORTED = [
     "cutoff2xi", "create_spme_engine", "destroy_spme_engine", "compute_spme",
     "spme_last_error", "spme_engine_create", "spme_engine_destroy",
     "spme_engine_set_positions", "spme_engine_apply", "spme_engine_info",
-    "spme_engine_debug_stage",
+    "spme_engine_debug_stage", "spme_engine_set_timing", "spme_engine_get_timing",
 ]
+
+STAGES = ["permute", "spread", "fft_x_fwd", "fft_y_fwd", "fft_z_infl",
+          "fft_y_inv", "fft_x_inv", "interp", "permute_real", "real"]
 
 
 class SpmeParams(ctypes.Structure):
@@ -68,6 +71,8 @@ def load() -> ctypes.CDLL:
     L.spme_engine_apply.argtypes = [vp, i, d, vp, l, d, vp, l, i, vp]
     L.spme_engine_info.argtypes = [vp, ctypes.POINTER(SpmeInfo)]
     L.spme_engine_debug_stage.argtypes = [vp, i, i, vp, l, vp]
+    L.spme_engine_set_timing.argtypes = [vp, i]
+    L.spme_engine_get_timing.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int)]
     _lib = L
     return L
 

@@ -66,10 +66,10 @@ def test_dropin_matches_oracle_c1(oracle, E, torch):
     d.close()
 
 
-@pytest.mark.parametrize("p", [4, 6, 8])
-def test_spread_and_fft_stages(oracle, E, torch, p):
-    _say(f"case: stages p={p} K=32 N=300")
-    N, K = 300, 32
+@pytest.mark.parametrize("p,K", [(4, 32), (6, 32), (8, 48)])
+def test_spread_and_fft_stages(oracle, E, torch, p, K):
+    _say(f"case: stages p={p} K={K} N=300")
+    N = 300
     pos, L = _system(77 + p, N, 0.15)
     xi = 0.6
     f = W.forces(77 + p, N)[0]
