@@ -104,6 +104,8 @@ class Engine:
         torch = _torch()
         if f.dim() == 1:
             f = f.view(1, -1)
+        if u is not None and u.dim() == 1:
+            u = u.view(1, -1)
         if u is None:
             u = torch.zeros_like(f)
             beta = 0.0

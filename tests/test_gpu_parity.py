@@ -160,23 +160,24 @@ def test_full_mobility_vs_oracle_and_direct(oracle, E, torch):
     _say("case: full M.f fp64/fp32 vs oracle spme_full and direct Ewald")
     N = 1000
     pos, L = _system(1001, N, 0.1)
-    rc = 5.0
-    xi = E.cutoff2xi(rc, 1e-9, L)
+    # rc = 10 and xi = 0.5: real-space tail ~1e-9, xi*h = 0.27 at K = 64, p = 8
+    rc, xi = 10.0, 0.5
     f = W.forces(1001, N, 2)
     ref = oracle.spme_full(xi, 64, 8, rc, L, pos, f)
-    for prec, tol, dt in [(E.SPME_FP64, FP64_TOL, torch.float64),
-                          (E.SPME_FP32, FP32_TOL, torch.float32)]:
+    rows = W.rows(3, N, 200)
+    direct = oracle.ewald_direct(pos, L, f[0], rows=rows)
+    # the oracle's own SPME is 3.2e-6 from the direct sum at these parameters
+    for prec, tol, dt, acc in [(E.SPME_FP64, FP64_TOL, torch.float64, 1e-5),
+                               (E.SPME_FP32, FP32_TOL, torch.float32, 1e-5)]:
         eng = E.Engine(xi, 64, 8, N, L, rc=rc, precision=prec, max_rhs=2)
         eng.set_positions(pos)
         u = eng.apply(torch.tensor(f, device="cuda", dtype=dt), parts=E.SPME_FULL)
         got = u.double().cpu().numpy()
         assert _rel(got, ref) < tol, prec
         eng.close()
-    rows = W.rows(3, N, 200)
-    direct = oracle.ewald_direct(pos, L, f[0], rows=rows)
-    mine = got[0].reshape(N, 3)[rows].reshape(-1)
-    err = np.linalg.norm(mine - direct) / np.linalg.norm(direct)
-    assert err < 1e-4, err
+        mine = got[0].reshape(N, 3)[rows].reshape(-1)
+        err = np.linalg.norm(mine - direct) / np.linalg.norm(direct)
+        assert err < acc, (prec, err)
 
 
 def test_deterministic(E, torch):
