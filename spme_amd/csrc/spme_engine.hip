@@ -148,6 +148,11 @@ struct _spme_t {
   int *r_key, *r_counts, *r_cursor, *r_off, *r_perm;
   double *r_pos;
   void *r_fs;
+  // real-space pair blocks (sliced ELL, 64 rows per slice)
+  int nslices;
+  int *r_cnt, *r_width, *r_size, *r_soff, *r_col;
+  void *r_blk;
+  long r_E, r_cap;
   // drop-in staging (FP64 host API)
   double *h_in, *h_out;
   // per-kernel timing: when enabled, events are recorded on the apply
@@ -218,7 +223,8 @@ void free_all(_spme_t *e) {
   void *ptrs[] = {e->grid, e->tab, e->tw, e->pos, e->base_tmp, e->key_tmp, e->counts,
                   e->cursor, e->boff, e->perm, e->pbase, e->err, e->w_tmp, e->pw, e->fs,
                   e->r_key, e->r_counts, e->r_cursor, e->r_off, e->r_perm, e->r_pos,
-                  e->r_fs, e->h_in, e->h_out};
+                  e->r_fs, e->r_cnt, e->r_width, e->r_size, e->r_soff, e->r_col, e->r_blk,
+                  e->h_in, e->h_out};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &r : e->pending) {
@@ -289,6 +295,11 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
     E_TRY(hipMalloc(&e->r_perm, sizeof(int) * N));
     E_TRY(hipMalloc(&e->r_pos, sizeof(double) * 3 * N));
     E_TRY(hipMalloc(&e->r_fs, e->tsz * (size_t)C * N));
+    e->nslices = (N + 63) / 64;
+    E_TRY(hipMalloc(&e->r_cnt, sizeof(int) * N));
+    E_TRY(hipMalloc(&e->r_width, sizeof(int) * e->nslices));
+    E_TRY(hipMalloc(&e->r_size, sizeof(int) * e->nslices));
+    E_TRY(hipMalloc(&e->r_soff, sizeof(int) * (e->nslices + 1)));
   }
 #undef E_TRY
   *out = e;
@@ -318,6 +329,29 @@ int build_P(_spme_t *e, hipStream_t st) {
     launch_pscatter(e->r_key, e->r_off, e->r_cursor, e->r_perm, N, st);
     launch_bucket_sort(e->r_off, e->r_perm, nc3, st);
     launch_cell_gather(e->r_perm, e->pos, e->r_pos, N, st);
+    const double L = e->prm.box_size, rc = e->prm.rc;
+    const double cut = rc > 2.0 ? rc : 2.0;
+    launch_pair_count(e->r_pos, e->r_off, e->nc, L, cut * cut, N, e->r_cnt, st);
+    launch_slice_width(e->r_cnt, N, e->r_width, e->r_size, st);
+    launch_scan(e->r_size, e->r_soff, e->nslices, st);
+    int total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, e->r_soff + e->nslices, sizeof(int), hipMemcpyDeviceToHost,
+                           st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (total > e->r_cap) {
+      (void)hipFree(e->r_col);
+      (void)hipFree(e->r_blk);
+      e->r_col = nullptr;
+      e->r_blk = nullptr;
+      e->r_cap = 0;
+      const long cap = total + total / 8 + 64;
+      HIP_TRY(hipMalloc(&e->r_col, sizeof(int) * cap));
+      HIP_TRY(hipMalloc(&e->r_blk, sizeof(T) * 6 * cap));
+      e->r_cap = cap;
+    }
+    e->r_E = e->r_cap;
+    launch_pair_fill<T>(e->r_pos, e->r_off, e->nc, L, e->prm.xi, rc * rc, cut * cut, N,
+                        e->r_width, e->r_soff, e->r_col, (T *)e->r_blk, e->r_E, st);
   }
   HIP_TRY(hipGetLastError());
   return 0;
@@ -366,8 +400,9 @@ int collect_timing(_spme_t *e) {
   return 0;
 }
 
-int pick_cc(int C, int P) {
-  if (P < 8 && C % 12 == 0) return 12;
+int pick_cc(int C, int P, size_t tsz) {
+  (void)P;
+  if (tsz == 4 && C % 12 == 0) return 12;
   return C % 6 == 0 ? 6 : 3;
 }
 
@@ -395,7 +430,7 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
   if (parts & SPME_RECIP) {
     launch_permute_in<T>(e->perm, in, ldin, (T *)e->fs, N, C, st);
     mark(SPME_ST_PERMUTE);
-    SpreadArgs<T> sa{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, C, pick_cc(C, e->P),
+    SpreadArgs<T> sa{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, C, pick_cc(C, e->P, e->tsz),
                      e->P, (T *)e->grid, e->comp_stride, e->RS, e->K,
                      e->K >= 64 ? e->K / 4 : e->K / 2};
     if (launch_spread<T>(sa, st)) return fail(-1, "spread: unsupported configuration");
@@ -411,7 +446,7 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
     if (timed)
       for (int i = 0; i < 5; ++i) rec.marks.push_back({SPME_ST_FFT_X_FWD + i, fev[i]});
     if (stop_after == 2) return 0;
-    InterpArgs<T> ia{e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid, e->comp_stride,
+    InterpArgs<T> ia{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid, e->comp_stride,
                      e->RS, e->K, N, C, e->P, T(alpha), T(beta), out, ldout};
     if (launch_interp<T>(ia, st)) return fail(-1, "interpolate: unsupported configuration");
     mark(SPME_ST_INTERP);
@@ -424,9 +459,8 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
                             ? 1.0 - 6.0 * xi / std::sqrt(M_PI) +
                                   40.0 * xi * xi * xi / (3.0 * std::sqrt(M_PI))
                             : 0.0;
-    const double rc2 = e->prm.rc * e->prm.rc;
-    RealArgs<T> ra{e->r_pos, e->r_perm, e->r_off, e->nc, e->prm.box_size, xi, rc2, self,
-                   (const T *)e->r_fs, N, C, alpha, beta, (parts & SPME_RECIP) ? 1 : 0,
+    RealArgs<T> ra{e->r_perm, e->r_width, e->r_soff, e->r_col, (const T *)e->r_blk, e->r_E,
+                   self, (const T *)e->r_fs, N, C, alpha, beta, (parts & SPME_RECIP) ? 1 : 0,
                    out, ldout};
     if (launch_real<T>(ra, st)) return fail(-1, "real space: unsupported batch width");
     mark(SPME_ST_REAL);

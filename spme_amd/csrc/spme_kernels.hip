@@ -138,9 +138,13 @@ __global__ __launch_bounds__(256) void k_permute_in(const int *__restrict__ perm
 // Spread (old-version/spme_kernel.c:231-312): grid[c] = sum_i q_i (x) f_ic.
 // One wave owns a TX x TY column of cells over a z range and keeps a window
 // of P planes per component in registers; particles are visited in
-// increasing base-z, and a plane is final (and stored) once the sweep has
-// passed every base that reaches it.  Every cell is written exactly once, so
-// the mesh needs no clearing and no atomics are used.
+// increasing base z, and the lowest plane of the window is final (and
+// stored) once the sweep has passed every base that reaches it; the window
+// then shifts by one plane.  Every cell is written exactly once, so the
+// mesh needs no clearing and no atomics are used.
+// Latency: the bucket offsets of the whole sweep are read into LDS up front,
+// and each step's particles (base, weights, forces) are staged into LDS with
+// one batch of independent loads.
 template <typename T, int P, int CC, int TX, int TY>
 __global__ __launch_bounds__(64) void k_spread(const int *__restrict__ boff,
                                                const int *__restrict__ pbase,
@@ -151,6 +155,15 @@ __global__ __launch_bounds__(64) void k_spread(const int *__restrict__ boff,
   static_assert(TX * TY == 64, "one wave per tile");
   constexpr int NXR = 1 + (P + TX - 2) / TX;
   constexpr int NYR = 1 + (P + TY - 2) / TY;
+  constexpr int NTL = NXR * NYR;
+  constexpr int MAXST = 256 + P;  // >= ZC + P - 1
+  constexpr int MAXB = 16;        // particles staged per batch
+  constexpr int NW = 3 * P;
+  __shared__ int s_lo[NTL * MAXST];
+  __shared__ int s_n[NTL * MAXST];
+  __shared__ int s_pb[MAXB];
+  __shared__ T s_w[MAXB * NW];
+  __shared__ T s_f[MAXB * CC];
   const int ntx = K / TX, nty = K / TY;
   const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
   const int lane = threadIdx.x;
@@ -160,124 +173,307 @@ __global__ __launch_bounds__(64) void k_spread(const int *__restrict__ boff,
   const int zs = z0 - (P - 1);
   const int nsteps = zend - zs;
 
+  for (int q = lane; q < NTL * nsteps; q += 64) {
+    const int t = q / nsteps, st = q - t * nsteps;
+    int tyy = ty - t / NXR, txx = tx - t % NXR;
+    tyy += tyy < 0 ? nty : 0;
+    txx += txx < 0 ? ntx : 0;
+    int bz = zs + st;
+    bz += bz < 0 ? K : 0;
+    const int key = (tyy * ntx + txx) * K + bz;
+    const int lo = boff[key];
+    s_lo[q] = lo;
+    s_n[q] = boff[key + 1] - lo;
+  }
+  __syncthreads();
+
   T acc[P][CC];
 #pragma unroll
   for (int j = 0; j < P; ++j)
 #pragma unroll
     for (int c = 0; c < CC; ++c) acc[j][c] = T(0);
 
-  for (int s0 = 0; s0 < nsteps; s0 += P) {
+  for (int st = 0; st < nsteps; ++st) {
+    int cnt[NTL], lo[NTL];
+    int n = 0;
 #pragma unroll
-    for (int ph = 0; ph < P; ++ph) {
-      const int s = s0 + ph;
-      if (s < nsteps) {
-        int bz = zs + s;
-        if (bz < 0) bz += K;
+    for (int t = 0; t < NTL; ++t) {
+      cnt[t] = s_n[t * nsteps + st];
+      lo[t] = s_lo[t * nsteps + st];
+      n += cnt[t];
+    }
+    // sorted index of the k-th particle of this step
+    auto locate = [&](int k) {
+      int i = lo[0] + k;
 #pragma unroll
-        for (int a = 0; a < NYR; ++a) {
-          int tyy = ty - a;
-          if (tyy < 0) tyy += nty;
+      for (int t = 0; t < NTL - 1; ++t) {
+        if (k >= cnt[t]) {
+          k -= cnt[t];
+          i = lo[t + 1] + k;
+        } else {
+          break;
+        }
+      }
+      return i;
+    };
+    for (int b0 = 0; b0 < n; b0 += MAXB) {
+      const int nb = min(MAXB, n - b0);
+      // stage the batch: all loads first, then the LDS stores
+      constexpr int RW = (MAXB * NW + 63) / 64, RF = (MAXB * CC + 63) / 64;
+      int vpb = 0;
+      T vw[RW], vf[RF];
+      if (lane < nb) vpb = pbase[locate(b0 + lane)];
 #pragma unroll
-          for (int b = 0; b < NXR; ++b) {
-            int txx = tx - b;
-            if (txx < 0) txx += ntx;
-            const int key = (tyy * ntx + txx) * K + bz;
-            const int i0 = boff[key], i1 = boff[key + 1];
-            for (int i = i0; i < i1; ++i) {
-              const int pb = pbase[i];
-              int dx = x - (pb & 1023);
-              dx += dx < 0 ? K : 0;
-              int dy = y - ((pb >> 10) & 1023);
-              dy += dy < 0 ? K : 0;
-              if (dx < P && dy < P) {
-                const T *w = pw + (size_t)i * 3 * P;
-                T wx = w[0], wy = w[P];
+      for (int r = 0; r < RW; ++r) {
+        const int g = lane + 64 * r;
+        vw[r] = g < nb * NW ? pw[(size_t)locate(b0 + g / NW) * NW + g % NW] : T(0);
+      }
 #pragma unroll
-                for (int j = 1; j < P; ++j) {
-                  wx = dx == j ? w[j] : wx;
-                  wy = dy == j ? w[P + j] : wy;
-                }
-                const T wxy = wx * wy;
-                T wf[CC];
+      for (int r = 0; r < RF; ++r) {
+        const int g = lane + 64 * r;
+        vf[r] = g < nb * CC ? fs[(size_t)locate(b0 + g / CC) * C + cc0 + g % CC] : T(0);
+      }
+      if (lane < nb) s_pb[lane] = vpb;
 #pragma unroll
-                for (int c = 0; c < CC; ++c) wf[c] = wxy * fs[(size_t)i * C + cc0 + c];
+      for (int r = 0; r < RW; ++r)
+        if (lane + 64 * r < MAXB * NW) s_w[lane + 64 * r] = vw[r];
 #pragma unroll
-                for (int j = 0; j < P; ++j) {
-                  const T wz = w[2 * P + j];
+      for (int r = 0; r < RF; ++r)
+        if (lane + 64 * r < MAXB * CC) s_f[lane + 64 * r] = vf[r];
+      __syncthreads();
+      for (int k = 0; k < nb; ++k) {
+        const int pb = s_pb[k];
+        int dx = x - (pb & 1023);
+        dx += dx < 0 ? K : 0;
+        int dy = y - ((pb >> 10) & 1023);
+        dy += dy < 0 ? K : 0;
+        if (dx < P && dy < P) {
+          const T *w = s_w + k * NW;
+          const T wxy = w[dx] * w[P + dy];
+          T wf[CC];
 #pragma unroll
-                  for (int c = 0; c < CC; ++c) acc[(ph + j) % P][c] += wz * wf[c];
-                }
-              }
-            }
+          for (int c = 0; c < CC; ++c) wf[c] = wxy * s_f[k * CC + c];
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const T wz = w[2 * P + j];
+#pragma unroll
+            for (int c = 0; c < CC; ++c) acc[j][c] += wz * wf[c];
           }
         }
-        if (s >= P - 1) {
-          T *g = grid + ((size_t)(zs + s) * K + y) * RS + x + (size_t)cc0 * comp_stride;
-#pragma unroll
-          for (int c = 0; c < CC; ++c) g[(size_t)c * comp_stride] = acc[ph][c];
-        }
-#pragma unroll
-        for (int c = 0; c < CC; ++c) acc[ph][c] = T(0);
       }
+      __syncthreads();
     }
+    if (st >= P - 1) {
+      T *g = grid + ((size_t)(zs + st) * K + y) * RS + x + (size_t)cc0 * comp_stride;
+#pragma unroll
+      for (int c = 0; c < CC; ++c) g[(size_t)c * comp_stride] = acc[0][c];
+    }
+#pragma unroll
+    for (int j = 0; j < P - 1; ++j)
+#pragma unroll
+      for (int c = 0; c < CC; ++c) acc[j][c] = acc[j + 1][c];
+#pragma unroll
+    for (int c = 0; c < CC; ++c) acc[P - 1][c] = T(0);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Interpolation (old-version/spme_kernel.c:8-70):
 //   out[r*ldout + 3*i + d] = beta*out + alpha * sum_j q_i(j) grid[3r+d](j).
-// One wave per particle; lane (dx, dy) walks the P planes of its stencil
-// column, then the wave reduces.
-template <typename T, int P, int CC>
-__global__ __launch_bounds__(256) void k_interp(const int *__restrict__ pbase,
+// A block owns a TXY x TXY tile of stencil bases over a z range and a chunk
+// of CC components.  It streams the tile plus its (P-1)-wide halo through
+// LDS one z plane at a time (P+1 slots: the P planes the current particles
+// need and the one being filled); each particle is handled by exactly one
+// wave (lanes = the P x P stencil columns) and written without atomics.
+// The loads of step s+1 (its new plane and its particles) are issued before
+// step s is processed, so each step costs one barrier and no exposed
+// global-memory latency.
+template <typename T, int P, int CC, int TXY>
+__global__ __launch_bounds__(256) void k_interp(const int *__restrict__ boff,
+                                                const int *__restrict__ pbase,
                                                 const T *__restrict__ pw,
                                                 const int *__restrict__ perm,
                                                 const T *__restrict__ grid, long comp_stride,
-                                                int RS, int K, int N, int C, T alpha, T beta,
+                                                int RS, int K, int C, int ZC, T alpha, T beta,
                                                 T *__restrict__ out, long ldout) {
-  const int lane = threadIdx.x & 63;
-  const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (s >= N) return;
-  const int pb = __builtin_amdgcn_readfirstlane(pbase[s]);
-  const int bx = pb & 1023, by = (pb >> 10) & 1023, bz = pb >> 20;
+  constexpr int E = TXY + P - 1;
+  constexpr int EX = E | 1;  // odd row pitch
+  constexpr int PLANE = CC * E * EX;
+  constexpr int NSLOT = P + 1;
+  constexpr int NSUB = TXY / 8;  // sort tiles are 8 x 8
+  constexpr int NS2 = NSUB * NSUB;
+  constexpr int MAXST = 256;
+  constexpr int MAXB = 32;  // particles staged per batch
+  constexpr int NW = 3 * P;
+  constexpr int NPL = (CC * E * E + 255) / 256;  // plane values per thread
+  constexpr int RW = (MAXB * NW + 255) / 256;    // weights per thread
+  __shared__ T win[NSLOT * PLANE];
+  __shared__ int s_lo[NS2 * MAXST];
+  __shared__ int s_n[NS2 * MAXST];
+  __shared__ int s_pb[2][MAXB], s_perm[2][MAXB];
+  __shared__ T s_w[2][MAXB * NW];
+  const int nt = K / TXY;
+  const int tx = blockIdx.x % nt, ty = blockIdx.x / nt;
+  const int x0 = tx * TXY, y0 = ty * TXY;
+  const int z0 = blockIdx.y * ZC, zend = min(z0 + ZC, K);
+  const int nst = zend - z0;
+  const int cc0 = blockIdx.z * CC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const T *g0 = grid + (size_t)cc0 * comp_stride;
+
+  for (int q = tid; q < NS2 * nst; q += 256) {
+    const int sq = q / nst, st = q - sq * nst;
+    const int sx = tx * NSUB + sq % NSUB, sy = ty * NSUB + sq / NSUB;
+    const int key = (sy * (K / 8) + sx) * K + z0 + st;
+    const int lo = boff[key];
+    s_lo[q] = lo;
+    s_n[q] = boff[key + 1] - lo;
+  }
+  // per-thread plane offsets, fixed over the sweep: source offset within a
+  // mesh z plane (components of the chunk included) and LDS offset within a
+  // window slot; -1 past the plane
+  long src_off[NPL];
+  int dst_off[NPL];
+#pragma unroll
+  for (int r = 0; r < NPL; ++r) {
+    const int q = tid + 256 * r;
+    const int c = q / (E * E), rr = q - c * (E * E);
+    const int ey = rr / E, ex = rr - ey * E;
+    int yy = y0 + ey, xx = x0 + ex;
+    yy -= yy >= K ? K : 0;
+    xx -= xx >= K ? K : 0;
+    src_off[r] = (long)c * comp_stride + (long)yy * RS + xx;
+    dst_off[r] = q < CC * E * E ? (c * E + ey) * EX + ex : -1;
+  }
+  const long zstride = (long)K * RS;
+  for (int j = 0; j < P - 1; ++j) {
+    const T *gz = g0 + ((z0 + j) % K) * zstride;
+    T *wz = win + ((z0 + j) % NSLOT) * PLANE;
+#pragma unroll
+    for (int r = 0; r < NPL; ++r)
+      if (dst_off[r] >= 0) wz[dst_off[r]] = gz[src_off[r]];
+  }
+  __syncthreads();
+
+  // registers carrying the next step's loads
+  T vp[NPL], vw[RW];
+  int vpb = 0, vpm = 0, nb_next = 0;
+  // issue the loads of step st, particles [b0, b0 + MAXB) of it
+  auto fetch = [&](int st, int b0, bool with_plane) {
+    int cnt[NS2], lo[NS2];
+    int n = 0;
+#pragma unroll
+    for (int q = 0; q < NS2; ++q) {
+      cnt[q] = s_n[q * nst + st];
+      lo[q] = s_lo[q * nst + st];
+      n += cnt[q];
+    }
+    auto locate = [&](int k) {
+      int i = lo[0] + k;
+#pragma unroll
+      for (int q = 0; q < NS2 - 1; ++q) {
+        if (k >= cnt[q]) {
+          k -= cnt[q];
+          i = lo[q + 1] + k;
+        } else {
+          break;
+        }
+      }
+      return i;
+    };
+    const int nb = max(0, min(MAXB, n - b0));
+    if (with_plane) {
+      const T *gz = g0 + ((z0 + st + P - 1) % K) * zstride;
+#pragma unroll
+      for (int r = 0; r < NPL; ++r) vp[r] = dst_off[r] >= 0 ? gz[src_off[r]] : T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int g = tid + 256 * r;
+      vw[r] = g < nb * NW ? pw[(size_t)locate(b0 + g / NW) * NW + g % NW] : T(0);
+    }
+    if (tid < nb) {
+      const int i = locate(b0 + tid);
+      vpb = pbase[i];
+      vpm = perm[i];
+    }
+    nb_next = nb;
+    return n;
+  };
+  auto commit = [&](int st, int buf, bool with_plane) {
+    if (with_plane) {
+      T *wz = win + ((z0 + st + P - 1) % NSLOT) * PLANE;
+#pragma unroll
+      for (int r = 0; r < NPL; ++r)
+        if (dst_off[r] >= 0) wz[dst_off[r]] = vp[r];
+    }
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+      if (tid + 256 * r < MAXB * NW) s_w[buf][tid + 256 * r] = vw[r];
+    if (tid < MAXB) {
+      s_pb[buf][tid] = vpb;
+      s_perm[buf][tid] = vpm;
+    }
+  };
+
   const int dx = lane % P, dy = lane / P;
   const bool act = lane < P * P;
-  int x = bx + dx;
-  if (x >= K) x -= K;
-  int y = by + dy;
-  if (y >= K) y -= K;
-  const T *w = pw + (size_t)s * 3 * P;
-  const T wxy = act ? w[dx] * w[P + dy] : T(0);
-  const int o = __builtin_amdgcn_readfirstlane(perm[s]);
-  for (int c0 = 0; c0 < C; c0 += CC) {
-    T acc[CC];
+  int n_cur = fetch(0, 0, true);
+  int b_cur = 0, st = 0, buf = 0;
+  while (st < nst) {
+    const int bz = z0 + st;
+    const int slot0 = bz % NSLOT;
+    const int nb = nb_next;
+    commit(st, buf, b_cur == 0);
+    __syncthreads();
+    // next unit of work: the rest of this step, or the next step
+    const int b_nxt = b_cur + MAXB < n_cur ? b_cur + MAXB : 0;
+    const int st_nxt = b_nxt ? st : st + 1;
+    int n_nxt = n_cur;
+    if (st_nxt < nst) n_nxt = fetch(st_nxt, b_nxt, b_nxt == 0);
+    for (int k = wave; k < nb; k += 4) {
+      const int pb = s_pb[buf][k];
+      const int rx = (pb & 1023) - x0, ry = ((pb >> 10) & 1023) - y0;
+      const T *w = s_w[buf] + k * NW;
+      const T wxy = act ? w[dx] * w[P + dy] : T(0);
+      T acc[CC];
 #pragma unroll
-    for (int c = 0; c < CC; ++c) acc[c] = T(0);
-    if (act) {
+      for (int c = 0; c < CC; ++c) acc[c] = T(0);
+      if (act) {
+        const int off = (ry + dy) * EX + rx + dx;
 #pragma unroll
-      for (int j = 0; j < P; ++j) {
-        int z = bz + j;
-        if (z >= K) z -= K;
-        const T wz = w[2 * P + j];
-        const T *g = grid + ((size_t)z * K + y) * RS + x + (size_t)c0 * comp_stride;
+        for (int j = 0; j < P; ++j) {
+          const T wz = w[2 * P + j];
+          const int sl = slot0 + j >= NSLOT ? slot0 + j - NSLOT : slot0 + j;
+          const T *pl = win + sl * PLANE + off;
 #pragma unroll
-        for (int c = 0; c < CC; ++c) acc[c] += wz * g[(size_t)c * comp_stride];
+          for (int c = 0; c < CC; ++c) acc[c] += wz * pl[c * E * EX];
+        }
+      }
+      const int o = s_perm[buf][k];
+#pragma unroll
+      for (int c = 0; c < CC; ++c) {
+        const T v = wave_sum(acc[c] * wxy);
+        if (lane == 0) {
+          const int comp = cc0 + c, r = comp / 3, d = comp - 3 * r;
+          T *u = out + r * ldout + 3L * o + d;
+          *u = beta == T(0) ? alpha * v : beta * *u + alpha * v;
+        }
       }
     }
-#pragma unroll
-    for (int c = 0; c < CC; ++c) {
-      const T v = wave_sum(acc[c] * wxy);
-      if (lane == 0) {
-        const int comp = c0 + c, r = comp / 3, d = comp - 3 * r;
-        T *u = out + r * ldout + 3L * o + d;
-        *u = beta == T(0) ? alpha * v : beta * *u + alpha * v;
-      }
-    }
+    b_cur = b_nxt;
+    st = st_nxt;
+    n_cur = n_nxt;
+    buf ^= 1;
   }
 }
 
 // ---------------------------------------------------------------------------
-// FFT passes.  tw[t] = exp(-2 pi i t/K), t < K.
+// FFT passes: four-step transforms of length K = N1 * N2 (Split<K>).  Step A:
+// per n2, an N1-point FFT over x[n1 N2 + n2] in registers, times
+// W_K^{n2 k1}, stored at k1 N2 + n2.  Step B: per k1, an N2-point FFT over
+// that contiguous block in registers gives X[k1 + N1 k2].  One LDS transpose
+// per transform.  tw[t] = exp(-2 pi i t/K), t < K.
 
 template <typename T, int K>
 __device__ __forceinline__ void load_tw(cpx<T> *s_tw, const cpx<T> *tw) {
@@ -288,8 +484,10 @@ __device__ __forceinline__ void load_tw(cpx<T> *s_tw, const cpx<T> *tw) {
 template <typename T, int K, int NL, int NT>
 __global__ __launch_bounds__(NT) void k_fft_x_r2c(T *__restrict__ grid, int RS, long npairs,
                                                   const cpx<T> *__restrict__ tw) {
+  constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
   constexpr int Kh = K / 2 + 1;
-  __shared__ cpx<T> buf[NL * K];
+  constexpr int LS = K + 1;
+  __shared__ cpx<T> buf[NL * LS];
   __shared__ cpx<T> s_tw[K];
   load_tw<T, K>(s_tw, tw);
   const long q0 = (long)blockIdx.x * NL;
@@ -301,16 +499,50 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(T *__restrict__ grid, int RS, 
       const T *ra = grid + (2 * q) * RS;
       v = {ra[n], ra[RS + n]};
     }
-    buf[t] = v;
+    buf[l * LS + n] = v;
   }
   __syncthreads();
-  lds_fft<T, K, -1, NL, K, NT>(buf, s_tw);
+  for (int t = threadIdx.x; t < NL * N2; t += NT) {  // step A, in place per thread
+    const int l = t / N2, n2 = t - l * N2;
+    cpx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) a[n1] = buf[l * LS + n1 * N2 + n2];
+    reg_fft<N1, -1>(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<-1>(s_tw, n2 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) buf[l * LS + k1 * N2 + n2] = a[k1];
+  }
+  __syncthreads();
+  constexpr int PB = (NL * N1 + NT - 1) / NT;
+  cpx<T> b[PB][N2];
+#pragma unroll
+  for (int u = 0; u < PB; ++u) {  // step B into registers
+    const int t = threadIdx.x + u * NT;
+    if (t < NL * N1) {
+      const int l = t / N1, k1 = t - l * N1;
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) b[u][n2] = buf[l * LS + k1 * N2 + n2];
+      reg_fft<N2, -1>(b[u]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PB; ++u) {  // natural order back into LDS
+    const int t = threadIdx.x + u * NT;
+    if (t < NL * N1) {
+      const int l = t / N1, k1 = t - l * N1;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) buf[l * LS + k1 + N1 * k2] = b[u][k2];
+    }
+  }
+  __syncthreads();
   for (int t = threadIdx.x; t < NL * Kh; t += NT) {
     const int l = t / Kh, k = t - l * Kh;
     const long q = q0 + l;
     if (q < npairs) {
-      const cpx<T> Z = buf[l * K + k];
-      const cpx<T> Zm = cconj(buf[l * K + (k == 0 ? 0 : K - k)]);
+      const cpx<T> Z = buf[l * LS + k];
+      const cpx<T> Zm = cconj(buf[l * LS + (k == 0 ? 0 : K - k)]);
       const cpx<T> A = {T(0.5) * (Z.x + Zm.x), T(0.5) * (Z.y + Zm.y)};
       const cpx<T> D = csub(Z, Zm);
       const cpx<T> B = {T(0.5) * D.y, T(-0.5) * D.x};
@@ -326,8 +558,10 @@ __global__ __launch_bounds__(NT) void k_fft_x_r2c(T *__restrict__ grid, int RS, 
 template <typename T, int K, int NL, int NT>
 __global__ __launch_bounds__(NT) void k_fft_x_c2r(T *__restrict__ grid, int RS, long npairs,
                                                   const cpx<T> *__restrict__ tw) {
+  constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
   constexpr int Kh = K / 2 + 1;
-  __shared__ cpx<T> buf[NL * K];
+  constexpr int LS = K + 1;
+  __shared__ cpx<T> buf[NL * LS];
   __shared__ cpx<T> s_tw[K];
   load_tw<T, K>(s_tw, tw);
   const long q0 = (long)blockIdx.x * NL;
@@ -344,19 +578,36 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(T *__restrict__ grid, int RS, 
       A.y = T(0);
       B.y = T(0);
     }
-    buf[l * K + k] = {A.x - B.y, A.y + B.x};
-    if (k > 0 && k < K / 2) buf[l * K + K - k] = {A.x + B.y, B.x - A.y};
+    buf[l * LS + k] = {A.x - B.y, A.y + B.x};
+    if (k > 0 && k < K / 2) buf[l * LS + K - k] = {A.x + B.y, B.x - A.y};
   }
   __syncthreads();
-  lds_fft<T, K, +1, NL, K, NT>(buf, s_tw);
-  for (int t = threadIdx.x; t < NL * K; t += NT) {
-    const int l = t / K, n = t - l * K;
+  for (int t = threadIdx.x; t < NL * N2; t += NT) {  // step A, in place per thread
+    const int l = t / N2, n2 = t - l * N2;
+    cpx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) a[n1] = buf[l * LS + n1 * N2 + n2];
+    reg_fft<N1, +1>(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<+1>(s_tw, n2 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) buf[l * LS + k1 * N2 + n2] = a[k1];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NL * N1; t += NT) {  // step B, straight to the rows
+    const int l = t / N1, k1 = t - l * N1;
     const long q = q0 + l;
+    cpx<T> b[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) b[n2] = buf[l * LS + k1 * N2 + n2];
+    reg_fft<N2, +1>(b);
     if (q < npairs) {
       T *ra = grid + (2 * q) * RS;
-      const cpx<T> v = buf[t];
-      ra[n] = v.x;
-      ra[RS + n] = v.y;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        ra[k1 + N1 * k2] = b[k2].x;
+        ra[RS + k1 + N1 * k2] = b[k2].y;
+      }
     }
   }
 }
@@ -365,27 +616,43 @@ __global__ __launch_bounds__(NT) void k_fft_x_c2r(T *__restrict__ grid, int RS, 
 template <typename T, int K, int XW, int S, int NT>
 __global__ __launch_bounds__(NT) void k_fft_y(T *__restrict__ grid, long comp_stride, int Khp,
                                               int nxc, const cpx<T> *__restrict__ tw) {
+  constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
   constexpr int Kh = K / 2 + 1;
   constexpr int LS = K + 1;
   __shared__ cpx<T> buf[XW * LS];
   __shared__ cpx<T> s_tw[K];
   load_tw<T, K>(s_tw, tw);
+  __syncthreads();
   const int xc = blockIdx.x % nxc;
   const int zc = blockIdx.x / nxc;  // z + K*comp
   const int comp = zc / K, z = zc - comp * K;
   const int x0 = xc * XW;
   cpx<T> *plane = reinterpret_cast<cpx<T> *>(grid + comp * comp_stride) + (size_t)z * K * Khp;
-  for (int t = threadIdx.x; t < XW * K; t += NT) {
-    const int y = t / XW, xo = t - y * XW;
+  for (int t = threadIdx.x; t < XW * N2; t += NT) {  // step A from global
+    const int xo = t % XW, n2 = t / XW;
     const int x = x0 + xo;
-    buf[xo * LS + y] = x < Kh ? plane[(size_t)y * Khp + x] : cpx<T>{T(0), T(0)};
+    cpx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1)
+      a[n1] = x < Kh ? plane[(size_t)(n1 * N2 + n2) * Khp + x] : cpx<T>{T(0), T(0)};
+    reg_fft<N1, S>(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<S>(s_tw, n2 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) buf[xo * LS + k1 * N2 + n2] = a[k1];
   }
   __syncthreads();
-  lds_fft<T, K, S, XW, LS, NT>(buf, s_tw);
-  for (int t = threadIdx.x; t < XW * K; t += NT) {
-    const int y = t / XW, xo = t - y * XW;
+  for (int t = threadIdx.x; t < XW * N1; t += NT) {  // step B to global
+    const int xo = t % XW, k1 = t / XW;
     const int x = x0 + xo;
-    if (x < Kh) plane[(size_t)y * Khp + x] = buf[xo * LS + y];
+    cpx<T> b[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) b[n2] = buf[xo * LS + k1 * N2 + n2];
+    reg_fft<N2, S>(b);
+    if (x < Kh) {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) plane[(size_t)(k1 + N1 * k2) * Khp + x] = b[k2];
+    }
   }
 }
 
@@ -393,14 +660,16 @@ __global__ __launch_bounds__(NT) void k_fft_y(T *__restrict__ grid, long comp_st
 // backward FFT along z, for the 3 components of one right-hand side.
 // Block = (x chunk, y) x rhs.  tab[(z*K + y)*Khp + x] = psi(|k|)|b(k)|^2/V.
 // Wave numbers follow old-version/spme.c:95-107 (index K/2 -> +pi K/L).
+// The spectrum of a (column, k1) stays in registers between the forward
+// N2-point FFT, the 3x3 influence and the backward N2-point FFT.
 template <typename T, int K, int XW, int NT>
 __global__ __launch_bounds__(NT) void k_fft_z_infl(T *__restrict__ grid, long comp_stride,
                                                    int Khp, int nxc, const T *__restrict__ tab,
                                                    T dk, const cpx<T> *__restrict__ tw) {
+  constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
   constexpr int Kh = K / 2 + 1;
   constexpr int LS = K + 1;
-  constexpr int NLN = 3 * XW;
-  __shared__ cpx<T> buf[NLN * LS];
+  __shared__ cpx<T> buf[3 * XW * LS];
   __shared__ cpx<T> s_tw[K];
   __shared__ T s_tab[XW * K];
   load_tw<T, K>(s_tw, tw);
@@ -412,49 +681,75 @@ __global__ __launch_bounds__(NT) void k_fft_z_infl(T *__restrict__ grid, long co
     const int x = x0 + xo;
     s_tab[xo * K + z] = x < Kh ? tab[((size_t)z * K + y) * Khp + x] : T(0);
   }
+  __syncthreads();
   cpx<T> *g0 = reinterpret_cast<cpx<T> *>(grid + (3L * rhs) * comp_stride);
   const long cs = comp_stride / 2;  // complex elements
-  for (int t = threadIdx.x; t < 3 * XW * K; t += NT) {
-    const int c = t / (XW * K), r = t - c * XW * K;
-    const int z = r / XW, xo = r - z * XW;
+  for (int t = threadIdx.x; t < 3 * XW * N2; t += NT) {  // step A (forward) from global
+    const int xo = t % XW, r = t / XW;
+    const int c = r % 3, n2 = r / 3;
     const int x = x0 + xo;
-    buf[(c * XW + xo) * LS + z] =
-        x < Kh ? g0[c * cs + ((size_t)z * K + y) * Khp + x] : cpx<T>{T(0), T(0)};
+    cpx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1)
+      a[n1] = x < Kh ? g0[c * cs + ((size_t)(n1 * N2 + n2) * K + y) * Khp + x]
+                     : cpx<T>{T(0), T(0)};
+    reg_fft<N1, -1>(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<-1>(s_tw, n2 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) buf[(c * XW + xo) * LS + k1 * N2 + n2] = a[k1];
   }
   __syncthreads();
-  lds_fft<T, K, -1, NLN, LS, NT>(buf, s_tw);
   const T ky = T(y <= K / 2 ? y : y - K) * dk;
-  for (int t = threadIdx.x; t < XW * K; t += NT) {
-    const int xo = t / K, z = t - xo * K;
+  for (int t = threadIdx.x; t < XW * N1; t += NT) {  // step B, influence, inverse step B
+    const int xo = t % XW, k1 = t / XW;
     const int x = x0 + xo;
-    const T kx = T(x) * dk;  // x <= K/2 in the half spectrum
-    const T kz = T(z <= K / 2 ? z : z - K) * dk;
-    const T k2 = kx * kx + ky * ky + kz * kz;
-    cpx<T> *a = buf + (0 * XW + xo) * LS + z;
-    cpx<T> *b = buf + (1 * XW + xo) * LS + z;
-    cpx<T> *c = buf + (2 * XW + xo) * LS + z;
-    if (k2 == T(0) || x >= Kh) {
-      *a = {T(0), T(0)};
-      *b = {T(0), T(0)};
-      *c = {T(0), T(0)};
-      continue;
+    cpx<T> b[3][N2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) b[c][n2] = buf[(c * XW + xo) * LS + k1 * N2 + n2];
+      reg_fft<N2, -1>(b[c]);
     }
-    const T m2 = s_tab[xo * K + z];
-    const T ik = T(1) / k2;
-    const T B0 = m2 - m2 * kx * kx * ik, B1 = -m2 * kx * ky * ik, B2 = -m2 * kx * kz * ik;
-    const T B3 = m2 - m2 * ky * ky * ik, B4 = -m2 * ky * kz * ik, B5 = m2 - m2 * kz * kz * ik;
-    const cpx<T> va = *a, vb = *b, vc = *c;
-    *a = {B0 * va.x + B1 * vb.x + B2 * vc.x, B0 * va.y + B1 * vb.y + B2 * vc.y};
-    *b = {B1 * va.x + B3 * vb.x + B4 * vc.x, B1 * va.y + B3 * vb.y + B4 * vc.y};
-    *c = {B2 * va.x + B4 * vb.x + B5 * vc.x, B2 * va.y + B4 * vb.y + B5 * vc.y};
+    const T kx = T(x) * dk;  // x <= K/2 in the half spectrum
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const int z = k1 + N1 * k2;
+      const T kz = T(z <= K / 2 ? z : z - K) * dk;
+      const T k2n = kx * kx + ky * ky + kz * kz;
+      const T m2 = s_tab[xo * K + z];
+      // k = 0 and padding columns carry m2 = 0; guard the division there
+      const T ik = k2n > T(0) ? T(1) / k2n : T(0);
+      const T B0 = m2 - m2 * kx * kx * ik, B1 = -m2 * kx * ky * ik, B2 = -m2 * kx * kz * ik;
+      const T B3 = m2 - m2 * ky * ky * ik, B4 = -m2 * ky * kz * ik, B5 = m2 - m2 * kz * kz * ik;
+      const cpx<T> va = b[0][k2], vb = b[1][k2], vc = b[2][k2];
+      b[0][k2] = {B0 * va.x + B1 * vb.x + B2 * vc.x, B0 * va.y + B1 * vb.y + B2 * vc.y};
+      b[1][k2] = {B1 * va.x + B3 * vb.x + B4 * vc.x, B1 * va.y + B3 * vb.y + B4 * vc.y};
+      b[2][k2] = {B2 * va.x + B4 * vb.x + B5 * vc.x, B2 * va.y + B4 * vb.y + B5 * vc.y};
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      reg_fft<N2, +1>(b[c]);
+#pragma unroll
+      for (int n2 = 1; n2 < N2; ++n2) b[c][n2] = cmul(b[c][n2], twiddle<+1>(s_tw, n2 * k1));
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) buf[(c * XW + xo) * LS + k1 * N2 + n2] = b[c][n2];
+    }
   }
   __syncthreads();
-  lds_fft<T, K, +1, NLN, LS, NT>(buf, s_tw);
-  for (int t = threadIdx.x; t < 3 * XW * K; t += NT) {
-    const int c = t / (XW * K), r = t - c * XW * K;
-    const int z = r / XW, xo = r - z * XW;
+  for (int t = threadIdx.x; t < 3 * XW * N2; t += NT) {  // inverse step A to global
+    const int xo = t % XW, r = t / XW;
+    const int c = r % 3, n2 = r / 3;
     const int x = x0 + xo;
-    if (x < Kh) g0[c * cs + ((size_t)z * K + y) * Khp + x] = buf[(c * XW + xo) * LS + z];
+    cpx<T> a[N1];
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) a[k1] = buf[(c * XW + xo) * LS + k1 * N2 + n2];
+    reg_fft<N1, +1>(a);
+    if (x < Kh) {
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1)
+        g0[c * cs + ((size_t)(n2 + N2 * n1) * K + y) * Khp + x] = a[n1];
+    }
   }
 }
 
@@ -508,16 +803,13 @@ static void spread_cc(const SpreadArgs<T> &a, hipStream_t st) {
                      a.boff, a.pbase, a.pw, a.fs, a.C, a.grid, a.comp_stride, a.RS, a.K, a.ZC);
 }
 
-// Components per pass: the register window holds P*CC values per lane;
-// P = 8 with 12 components no longer unrolls into registers (it spills).
+// Components per pass: the register window holds P*CC values per lane.
 template <typename T, int P>
 static int spread_p(const SpreadArgs<T> &a, hipStream_t st) {
   if (a.CC == 3) spread_cc<T, P, 3>(a, st);
   else if (a.CC == 6) spread_cc<T, P, 6>(a, st);
-  else if constexpr (P < 8) {
-    if (a.CC == 12) spread_cc<T, P, 12>(a, st);
-    else return -1;
-  } else return -1;
+  else if (a.CC == 12) spread_cc<T, P, 12>(a, st);
+  else return -1;
   return 0;
 }
 
@@ -532,19 +824,23 @@ int launch_spread(const SpreadArgs<T> &a, hipStream_t st) {
   return -1;
 }
 
-template <typename T, int P, int CC>
+// Interpolation shape: FP32 takes 3 components per block over a 16 x 16
+// tile; FP64 takes 1 (LDS: P planes x CC x (TXY+P-1)^2 values).
+template <typename T, int P, int CC, int TXY>
 static void interp_cc(const InterpArgs<T> &a, hipStream_t st) {
-  hipLaunchKernelGGL((k_interp<T, P, CC>), dim3(grid1(a.N, 4)), dim3(256), 0, st, a.pbase, a.pw,
-                     a.perm, a.grid, a.comp_stride, a.RS, a.K, a.N, a.C, a.alpha, a.beta,
-                     a.out, a.ldout);
+  const int nt = a.K / TXY;
+  const int zc = a.K >= 64 ? a.K / 4 : a.K;
+  hipLaunchKernelGGL((k_interp<T, P, CC, TXY>), dim3(nt * nt, (a.K + zc - 1) / zc, a.C / CC),
+                     dim3(256), 0, st, a.boff, a.pbase, a.pw, a.perm, a.grid, a.comp_stride,
+                     a.RS, a.K, a.C, zc, a.alpha, a.beta, a.out, a.ldout);
 }
 
 template <typename T, int P>
 static int interp_p(const InterpArgs<T> &a, hipStream_t st) {
-  if (a.C % 12 == 0) interp_cc<T, P, 12>(a, st);
-  else if (a.C % 6 == 0) interp_cc<T, P, 6>(a, st);
-  else if (a.C % 3 == 0) interp_cc<T, P, 3>(a, st);
-  else return -1;
+  constexpr int CC = sizeof(T) == 4 ? 3 : 1;
+  if (a.C % CC) return -1;
+  if (a.K % 16 == 0) interp_cc<T, P, CC, 16>(a, st);
+  else interp_cc<T, P, CC, 8>(a, st);
   return 0;
 }
 
@@ -561,7 +857,7 @@ int launch_interp(const InterpArgs<T> &a, hipStream_t st) {
 // Line batch sizes: keep ~24 KB of LDS per FFT block.
 template <typename T> struct FftShape {
   static constexpr int NLX = sizeof(T) == 8 ? 8 : 16;  // complex lines per x block
-  static constexpr int XW = sizeof(T) == 8 ? 8 : 16;   // y-pass columns
+  static constexpr int XW = 16;                        // y-pass columns
   static constexpr int XWZ = sizeof(T) == 8 ? 4 : 8;   // z-pass columns (x3 comps)
 };
 

@@ -22,6 +22,7 @@ template <typename T> struct SpreadArgs {
 };
 
 template <typename T> struct InterpArgs {
+  const int *boff;
   const int *pbase;
   const T *pw;
   const int *perm;

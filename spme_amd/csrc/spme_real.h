@@ -5,11 +5,13 @@
 namespace spme {
 
 template <typename T> struct RealArgs {
-  const double *spos;  // positions in cell order [N][3]
-  const int *perm;     // cell order -> original index
-  const int *coff;     // cell offsets [nc^3 + 1]
-  int nc;
-  double L, xi, rc2, self;
+  const int *perm;   // cell order -> original index
+  const int *width;  // entries per row, per slice of 64 rows
+  const int *soff;   // slice offsets (entries)
+  const int *col;    // neighbour (cell order) or -1
+  const T *blk;      // 6 x E block values
+  long E;
+  double self;
   const T *fs;  // forces in cell order [N][C]
   int N, C;
   double alpha, beta;
@@ -22,6 +24,13 @@ void launch_cell_key(const double *pos, int N, double L, int nc, int *key, int *
                      hipStream_t st);
 void launch_cell_gather(const int *perm, const double *pos, double *spos, int N,
                         hipStream_t st);
+void launch_pair_count(const double *spos, const int *coff, int nc, double L, double cut2,
+                       int N, int *cnt, hipStream_t st);
+void launch_slice_width(const int *cnt, int N, int *width, int *size, hipStream_t st);
+template <typename T>
+void launch_pair_fill(const double *spos, const int *coff, int nc, double L, double xi,
+                      double rc2, double cut2, int N, const int *width, const int *soff,
+                      int *col, T *blk, long E, hipStream_t st);
 template <typename T> int launch_real(const RealArgs<T> &a, hipStream_t st);
 
 }  // namespace spme

@@ -1,7 +1,7 @@
 // Real-space RPY part of the Beenakker-Ewald mobility (absent from the
-// reference, which computes only the wave-space part): a matrix-free 3x3
-// block SpMV over a cell list, with the Rotne-Prager overlap correction and
-// the self term.  Radius a = 1, mobility in units of 1/(6 pi eta a).
+// reference, which computes only the wave-space part): a 3x3 block-sparse
+// SpMV whose blocks are built once per position set over a cell list, with
+// the Rotne-Prager overlap correction and the self term.  Radius a = 1, mobility in units of 1/(6 pi eta a).
 //
 //   M_real(r) = F(r) I + G(r) r r^T / r^2                (Beenakker 1986)
 //   F = erfc(xi r)(3/(4r) + 1/(2r^3))
@@ -74,67 +74,143 @@ __global__ __launch_bounds__(256) void k_cell_gather(const int *__restrict__ per
   spos[3 * s + 2] = pos[3 * i + 2];
 }
 
-// One thread per particle (cell order).  fs: forces in cell order [N][C].
-// out[r*ldout + 3*perm[s] + d] = (accum ? out : beta*out) + alpha*(self f + sum M f).
-template <typename T, int C>
-__global__ __launch_bounds__(128) void k_real(const double *__restrict__ spos,
-                                              const int *__restrict__ perm,
-                                              const int *__restrict__ coff, int nc, double L,
-                                              double xi, double rc2, double self,
-                                              const T *__restrict__ fs, int N, double alpha,
-                                              double beta, int accum, T *__restrict__ out,
-                                              long ldout) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= N) return;
+// ---------------------------------------------------------------------------
+// Pair blocks, built once per position set.  Rows are particles in cell
+// order, grouped in slices of 64 rows; slice q holds width[q] entries per
+// row, stored entry-major (entry k of row 64q + l at soff[q] + 64k + l) so
+// that a wave reads each entry array coalesced.  Each entry is a neighbour
+// j (cell order) and the 6 unique values of its symmetric 3x3 block
+// M_real + overlap, or j = -1 past the end of a row.
+
+__device__ __forceinline__ int cell_of(double p, double L, int nc) {
+  double t = p / L;
+  t -= floor(t);
+  int v = int(t * nc);
+  return v >= nc ? nc - 1 : (v < 0 ? 0 : v);
+}
+
+template <typename F>
+__device__ __forceinline__ void for_neighbours(const double *__restrict__ spos,
+                                               const int *__restrict__ coff, int nc, double L,
+                                               int s, double cut2, F &&fn) {
   const double px = spos[3 * s], py = spos[3 * s + 1], pz = spos[3 * s + 2];
-  double acc[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) acc[c] = self * double(fs[(size_t)s * C + c]);
-  int cc[3];
-  {
-    double q[3] = {px / L, py / L, pz / L};
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      double t = q[d] - floor(q[d]);
-      int v = int(t * nc);
-      cc[d] = v >= nc ? nc - 1 : (v < 0 ? 0 : v);
-    }
-  }
+  const int cx0 = cell_of(px, L, nc), cy0 = cell_of(py, L, nc), cz0 = cell_of(pz, L, nc);
   const double iL = 1.0 / L;
   for (int dz = -1; dz <= 1; ++dz) {
-    int cz = cc[2] + dz;
+    int cz = cz0 + dz;
     cz += cz < 0 ? nc : (cz >= nc ? -nc : 0);
     for (int dy = -1; dy <= 1; ++dy) {
-      int cy = cc[1] + dy;
+      int cy = cy0 + dy;
       cy += cy < 0 ? nc : (cy >= nc ? -nc : 0);
       for (int dx = -1; dx <= 1; ++dx) {
-        int cx = cc[0] + dx;
+        int cx = cx0 + dx;
         cx += cx < 0 ? nc : (cx >= nc ? -nc : 0);
         const int cell = (cz * nc + cy) * nc + cx;
-        const int j0 = coff[cell], j1 = coff[cell + 1];
-        for (int j = j0; j < j1; ++j) {
+        for (int j = coff[cell], j1 = coff[cell + 1]; j < j1; ++j) {
           if (j == s) continue;
           double ex = px - spos[3 * j], ey = py - spos[3 * j + 1], ez = pz - spos[3 * j + 2];
           ex -= L * rint(ex * iL);
           ey -= L * rint(ey * iL);
           ez -= L * rint(ez * iL);
           const double r2 = ex * ex + ey * ey + ez * ez;
-          if (r2 >= rc2 && r2 >= 4.0) continue;
-          if (r2 == 0.0) continue;
-          double F, G;
-          pair_coeffs(r2, xi, rc2, F, G);
-          const double g = G / r2;
-          const T *fj = fs + (size_t)j * C;
-#pragma unroll
-          for (int r = 0; r < C / 3; ++r) {
-            const double f0 = fj[3 * r], f1 = fj[3 * r + 1], f2 = fj[3 * r + 2];
-            const double dot = g * (ex * f0 + ey * f1 + ez * f2);
-            acc[3 * r + 0] += F * f0 + ex * dot;
-            acc[3 * r + 1] += F * f1 + ey * dot;
-            acc[3 * r + 2] += F * f2 + ez * dot;
-          }
+          if (r2 >= cut2 || r2 == 0.0) continue;
+          fn(j, ex, ey, ez, r2);
         }
       }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pair_count(const double *__restrict__ spos,
+                                                    const int *__restrict__ coff, int nc,
+                                                    double L, double cut2, int N,
+                                                    int *__restrict__ cnt) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  int n = 0;
+  for_neighbours(spos, coff, nc, L, s, cut2,
+                 [&](int, double, double, double, double) { ++n; });
+  cnt[s] = n;
+}
+
+// slice q: size[q] = 64 * max count over its rows; width[q] = that max.
+__global__ __launch_bounds__(64) void k_slice_width(const int *__restrict__ cnt, int N,
+                                                    int *__restrict__ width,
+                                                    int *__restrict__ size) {
+  const int q = blockIdx.x, s = q * 64 + threadIdx.x;
+  int v = s < N ? cnt[s] : 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  if (threadIdx.x == 0) {
+    width[q] = v;
+    size[q] = 64 * v;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_pair_fill(const double *__restrict__ spos,
+                                                   const int *__restrict__ coff, int nc,
+                                                   double L, double xi, double rc2,
+                                                   double cut2, int N,
+                                                   const int *__restrict__ width,
+                                                   const int *__restrict__ soff,
+                                                   int *__restrict__ col, T *__restrict__ blk,
+                                                   long E) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const int q = s >> 6, l = s & 63;
+  const long base = soff[q] + l;
+  int k = 0;
+  for_neighbours(spos, coff, nc, L, s, cut2, [&](int j, double ex, double ey, double ez,
+                                                 double r2) {
+    double F, G;
+    pair_coeffs(r2, xi, rc2, F, G);
+    const double g = G / r2;
+    const long e = base + 64L * k;
+    col[e] = j;
+    blk[0 * E + e] = T(F + g * ex * ex);
+    blk[1 * E + e] = T(g * ex * ey);
+    blk[2 * E + e] = T(g * ex * ez);
+    blk[3 * E + e] = T(F + g * ey * ey);
+    blk[4 * E + e] = T(g * ey * ez);
+    blk[5 * E + e] = T(F + g * ez * ez);
+    ++k;
+  });
+  for (; k < width[q]; ++k) col[base + 64L * k] = -1;
+}
+
+// One thread per row (cell order).  fs: forces in cell order [N][C].
+// out[r*ldout + 3*perm[s] + d] = (accum ? out : beta*out) + alpha*(self f + sum_j B_sj f_j).
+template <typename T, int C>
+__global__ __launch_bounds__(256) void k_real_spmv(const int *__restrict__ perm,
+                                                   const int *__restrict__ width,
+                                                   const int *__restrict__ soff,
+                                                   const int *__restrict__ col,
+                                                   const T *__restrict__ blk, long E,
+                                                   T self, const T *__restrict__ fs, int N,
+                                                   T alpha, T beta, int accum,
+                                                   T *__restrict__ out, long ldout) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const int q = s >> 6, l = s & 63;
+  T acc[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) acc[c] = self * fs[(size_t)s * C + c];
+  const long base = soff[q] + l;
+  const int w = width[q];
+  for (int k = 0; k < w; ++k) {
+    const long e = base + 64L * k;
+    const int j = col[e];
+    if (j < 0) break;
+    const T b0 = blk[e], b1 = blk[E + e], b2 = blk[2 * E + e];
+    const T b3 = blk[3 * E + e], b4 = blk[4 * E + e], b5 = blk[5 * E + e];
+    const T *fj = fs + (size_t)j * C;
+#pragma unroll
+    for (int r = 0; r < C / 3; ++r) {
+      const T f0 = fj[3 * r], f1 = fj[3 * r + 1], f2 = fj[3 * r + 2];
+      acc[3 * r + 0] += b0 * f0 + b1 * f1 + b2 * f2;
+      acc[3 * r + 1] += b1 * f0 + b3 * f1 + b4 * f2;
+      acc[3 * r + 2] += b2 * f0 + b4 * f1 + b5 * f2;
     }
   }
   const int o = perm[s];
@@ -142,8 +218,8 @@ __global__ __launch_bounds__(128) void k_real(const double *__restrict__ spos,
   for (int c = 0; c < C; ++c) {
     const int r = c / 3, d = c - 3 * r;
     T *u = out + r * ldout + 3L * o + d;
-    const double prev = accum ? double(*u) : (beta == 0.0 ? 0.0 : beta * double(*u));
-    *u = T(prev + alpha * acc[c]);
+    const T prev = accum ? *u : (beta == T(0) ? T(0) : beta * *u);
+    *u = prev + alpha * acc[c];
   }
 }
 
@@ -160,11 +236,29 @@ void launch_cell_gather(const int *perm, const double *pos, double *spos, int N,
   hipLaunchKernelGGL(k_cell_gather, dim3(grid1(N, 256)), dim3(256), 0, st, perm, pos, spos, N);
 }
 
+void launch_pair_count(const double *spos, const int *coff, int nc, double L, double cut2,
+                       int N, int *cnt, hipStream_t st) {
+  hipLaunchKernelGGL(k_pair_count, dim3(grid1(N, 256)), dim3(256), 0, st, spos, coff, nc, L,
+                     cut2, N, cnt);
+}
+
+void launch_slice_width(const int *cnt, int N, int *width, int *size, hipStream_t st) {
+  hipLaunchKernelGGL(k_slice_width, dim3(grid1(N, 64)), dim3(64), 0, st, cnt, N, width, size);
+}
+
+template <typename T>
+void launch_pair_fill(const double *spos, const int *coff, int nc, double L, double xi,
+                      double rc2, double cut2, int N, const int *width, const int *soff,
+                      int *col, T *blk, long E, hipStream_t st) {
+  hipLaunchKernelGGL(k_pair_fill<T>, dim3(grid1(N, 256)), dim3(256), 0, st, spos, coff, nc, L,
+                     xi, rc2, cut2, N, width, soff, col, blk, E);
+}
+
 template <typename T, int C>
 static void real_c(const RealArgs<T> &a, hipStream_t st) {
-  hipLaunchKernelGGL((k_real<T, C>), dim3(grid1(a.N, 128)), dim3(128), 0, st, a.spos, a.perm,
-                     a.coff, a.nc, a.L, a.xi, a.rc2, a.self, a.fs, a.N, a.alpha, a.beta,
-                     a.accum, a.out, a.ldout);
+  hipLaunchKernelGGL((k_real_spmv<T, C>), dim3(grid1(a.N, 256)), dim3(256), 0, st, a.perm,
+                     a.width, a.soff, a.col, a.blk, a.E, T(a.self), a.fs, a.N, T(a.alpha),
+                     T(a.beta), a.accum, a.out, a.ldout);
 }
 
 template <typename T>
@@ -179,6 +273,12 @@ int launch_real(const RealArgs<T> &a, hipStream_t st) {
   return -1;
 }
 
+template void launch_pair_fill<float>(const double *, const int *, int, double, double, double,
+                                      double, int, const int *, const int *, int *, float *,
+                                      long, hipStream_t);
+template void launch_pair_fill<double>(const double *, const int *, int, double, double, double,
+                                       double, int, const int *, const int *, int *, double *,
+                                       long, hipStream_t);
 template int launch_real<float>(const RealArgs<float> &, hipStream_t);
 template int launch_real<double>(const RealArgs<double> &, hipStream_t);
 
