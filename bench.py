@@ -37,7 +37,38 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rhs", type=int, default=1)
+    ap.add_argument("--pmc-run", action="store_true",
+                    help="short run for hardware-counter passes: a plain copy of "
+                         "known size (calibration), then 2 batched applies")
     return ap.parse_args()
+
+
+def pmc_run():
+    """Workload for rocprofv3 --pmc passes.  The copy moves exactly 2^28
+    bytes each way so FETCH_SIZE/WRITE_SIZE can be scaled per byte."""
+    import torch
+    from spme_amd import engine as E
+    from spme_amd import workload as W
+    cfg = W.CONFIGS["c3"]
+    a = torch.empty(2 ** 26, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    torch.cuda.synchronize()
+    b.copy_(a)  # calibration: 256 MiB read, 256 MiB written
+    torch.cuda.synchronize()
+    del a, b
+    s = W.make_system(cfg["seed"], cfg["n"], cfg["phi"])
+    f = torch.tensor(W.forces(cfg["seed"], cfg["n"], cfg["nrhs"]), device="cuda:0",
+                     dtype=torch.float32)
+    eng = E.Engine(0.46, cfg["K"], cfg["p"], cfg["n"], s.L, rc=8.0,
+                   precision=E.SPME_FP32, max_rhs=cfg["nrhs"])
+    eng.set_positions(torch.tensor(s.pos, device="cuda:0"))
+    u = torch.zeros_like(f)
+    for _ in range(2):
+        eng.apply(f, u, parts=E.SPME_FULL)
+    torch.cuda.synchronize()
+    eng.close()
+    print("[pmc] done", file=sys.stderr, flush=True)
 
 
 def algorithmic_bytes(N, K, p, R, s):
@@ -67,6 +98,9 @@ def algorithmic_bytes(N, K, p, R, s):
 
 def main():
     args = parse()
+    if args.pmc_run:
+        pmc_run()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU "
                           "sharding is a design note only (see the design "

@@ -245,7 +245,9 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
   e->prec = prm->precision;
   e->max_rhs = prm->max_rhs;
   e->Kh = e->K / 2 + 1;
-  e->Khp = (e->Kh + 7) / 8 * 8;
+  // rows padded to whole 128-byte lines in FP32 (16 complex) so that the
+  // column chunks of the y and z passes read whole lines
+  e->Khp = (e->Kh + 15) / 16 * 16;
   e->RS = 2 * e->Khp;
   e->comp_stride = (long)e->K * e->K * e->RS;
   e->tsz = e->prec == SPME_FP32 ? 4 : 8;

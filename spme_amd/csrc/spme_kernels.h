@@ -132,6 +132,16 @@ __device__ __forceinline__ cpx<T> twiddle(const cpx<T> *tw, int t) {
   return w;
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8), each with its own L2.  xcd_remap(b, n) gives the logical index of
+// hardware block b such that every XCD processes one contiguous range of
+// logical blocks, so neighbouring blocks (which share halo data) share an L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  const int q = n >> 3, r = n & 7;
+  const int x = b & 7, i = b >> 3;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 // Wave64 sum through DPP lane moves (no LDS traffic): xor-1 and xor-2 quad
 // permutes, half-row and row mirrors (each 16-lane row then holds its sum),
 // then row_bcast:15 and row_bcast:31 fold the four rows into lane 63, which

@@ -165,11 +165,14 @@ __global__ __launch_bounds__(64) void k_spread(const int *__restrict__ boff,
   __shared__ T s_w[MAXB * NW];
   __shared__ T s_f[MAXB * CC];
   const int ntx = K / TX, nty = K / TY;
-  const int tx = blockIdx.x % ntx, ty = blockIdx.x / ntx;
+  const int lb = xcd_remap(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                           gridDim.x * gridDim.y * gridDim.z);
+  const int tile = lb % gridDim.x, rest = lb / gridDim.x;
+  const int tx = tile % ntx, ty = tile / ntx;
   const int lane = threadIdx.x;
   const int x = tx * TX + lane % TX, y = ty * TY + lane / TX;
-  const int z0 = blockIdx.y * ZC, zend = min(z0 + ZC, K);
-  const int cc0 = blockIdx.z * CC;
+  const int z0 = (rest % gridDim.y) * ZC, zend = min(z0 + ZC, K);
+  const int cc0 = (rest / gridDim.y) * CC;
   const int zs = z0 - (P - 1);
   const int nsteps = zend - zs;
 
@@ -288,8 +291,8 @@ __global__ __launch_bounds__(64) void k_spread(const int *__restrict__ boff,
 // The loads of step s+1 (its new plane and its particles) are issued before
 // step s is processed, so each step costs one barrier and no exposed
 // global-memory latency.
-template <typename T, int P, int CC, int TXY>
-__global__ __launch_bounds__(256) void k_interp(const int *__restrict__ boff,
+template <typename T, int P, int CC, int TXY, int NT>
+__global__ __launch_bounds__(NT) void k_interp(const int *__restrict__ boff,
                                                 const int *__restrict__ pbase,
                                                 const T *__restrict__ pw,
                                                 const int *__restrict__ perm,
@@ -302,26 +305,31 @@ __global__ __launch_bounds__(256) void k_interp(const int *__restrict__ boff,
   constexpr int NSLOT = P + 1;
   constexpr int NSUB = TXY / 8;  // sort tiles are 8 x 8
   constexpr int NS2 = NSUB * NSUB;
-  constexpr int MAXST = 256;
+  constexpr int MAXST = 64;  // >= ZC
+  constexpr int NWV = NT / 64;
   constexpr int MAXB = 32;  // particles staged per batch
   constexpr int NW = 3 * P;
-  constexpr int NPL = (CC * E * E + 255) / 256;  // plane values per thread
-  constexpr int RW = (MAXB * NW + 255) / 256;    // weights per thread
+  constexpr int NPL = (CC * E * E + NT - 1) / NT;  // plane values per thread
+  constexpr int RW = (MAXB * NW + NT - 1) / NT;    // weights per thread
   __shared__ T win[NSLOT * PLANE];
   __shared__ int s_lo[NS2 * MAXST];
   __shared__ int s_n[NS2 * MAXST];
   __shared__ int s_pb[2][MAXB], s_perm[2][MAXB];
   __shared__ T s_w[2][MAXB * NW];
   const int nt = K / TXY;
-  const int tx = blockIdx.x % nt, ty = blockIdx.x / nt;
+  // logical block: tiles fastest, so each XCD gets a band of adjacent tiles
+  const int lb = xcd_remap(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                           gridDim.x * gridDim.y * gridDim.z);
+  const int tile = lb % (nt * nt), rest = lb / (nt * nt);
+  const int tx = tile % nt, ty = tile / nt;
   const int x0 = tx * TXY, y0 = ty * TXY;
-  const int z0 = blockIdx.y * ZC, zend = min(z0 + ZC, K);
+  const int z0 = (rest % gridDim.y) * ZC, zend = min(z0 + ZC, K);
   const int nst = zend - z0;
-  const int cc0 = blockIdx.z * CC;
+  const int cc0 = (rest / gridDim.y) * CC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const T *g0 = grid + (size_t)cc0 * comp_stride;
 
-  for (int q = tid; q < NS2 * nst; q += 256) {
+  for (int q = tid; q < NS2 * nst; q += NT) {
     const int sq = q / nst, st = q - sq * nst;
     const int sx = tx * NSUB + sq % NSUB, sy = ty * NSUB + sq / NSUB;
     const int key = (sy * (K / 8) + sx) * K + z0 + st;
@@ -336,7 +344,7 @@ __global__ __launch_bounds__(256) void k_interp(const int *__restrict__ boff,
   int dst_off[NPL];
 #pragma unroll
   for (int r = 0; r < NPL; ++r) {
-    const int q = tid + 256 * r;
+    const int q = tid + NT * r;
     const int c = q / (E * E), rr = q - c * (E * E);
     const int ey = rr / E, ex = rr - ey * E;
     int yy = y0 + ey, xx = x0 + ex;
@@ -389,7 +397,7 @@ __global__ __launch_bounds__(256) void k_interp(const int *__restrict__ boff,
     }
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
-      const int g = tid + 256 * r;
+      const int g = tid + NT * r;
       vw[r] = g < nb * NW ? pw[(size_t)locate(b0 + g / NW) * NW + g % NW] : T(0);
     }
     if (tid < nb) {
@@ -409,7 +417,7 @@ __global__ __launch_bounds__(256) void k_interp(const int *__restrict__ boff,
     }
 #pragma unroll
     for (int r = 0; r < RW; ++r)
-      if (tid + 256 * r < MAXB * NW) s_w[buf][tid + 256 * r] = vw[r];
+      if (tid + NT * r < MAXB * NW) s_w[buf][tid + NT * r] = vw[r];
     if (tid < MAXB) {
       s_pb[buf][tid] = vpb;
       s_perm[buf][tid] = vpm;
@@ -431,7 +439,7 @@ __global__ __launch_bounds__(256) void k_interp(const int *__restrict__ boff,
     const int st_nxt = b_nxt ? st : st + 1;
     int n_nxt = n_cur;
     if (st_nxt < nst) n_nxt = fetch(st_nxt, b_nxt, b_nxt == 0);
-    for (int k = wave; k < nb; k += 4) {
+    for (int k = wave; k < nb; k += NWV) {
       const int pb = s_pb[buf][k];
       const int rx = (pb & 1023) - x0, ry = ((pb >> 10) & 1023) - y0;
       const T *w = s_w[buf] + k * NW;
@@ -623,8 +631,9 @@ __global__ __launch_bounds__(NT) void k_fft_y(T *__restrict__ grid, long comp_st
   __shared__ cpx<T> s_tw[K];
   load_tw<T, K>(s_tw, tw);
   __syncthreads();
-  const int xc = blockIdx.x % nxc;
-  const int zc = blockIdx.x / nxc;  // z + K*comp
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int xc = lb % nxc;
+  const int zc = lb / nxc;  // z + K*comp
   const int comp = zc / K, z = zc - comp * K;
   const int x0 = xc * XW;
   cpx<T> *plane = reinterpret_cast<cpx<T> *>(grid + comp * comp_stride) + (size_t)z * K * Khp;
@@ -673,9 +682,11 @@ __global__ __launch_bounds__(NT) void k_fft_z_infl(T *__restrict__ grid, long co
   __shared__ cpx<T> s_tw[K];
   __shared__ T s_tab[XW * K];
   load_tw<T, K>(s_tw, tw);
-  const int xc = blockIdx.x % nxc, y = blockIdx.x / nxc;
+  // adjacent x chunks (halves of one cache line) on the same XCD
+  const int lb = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+  const int xc = lb % nxc, y = (lb / nxc) % K;
   const int x0 = xc * XW;
-  const int rhs = blockIdx.y;
+  const int rhs = lb / (nxc * K);
   for (int t = threadIdx.x; t < XW * K; t += NT) {
     const int z = t / XW, xo = t - z * XW;
     const int x = x0 + xo;
@@ -828,10 +839,11 @@ int launch_spread(const SpreadArgs<T> &a, hipStream_t st) {
 // tile; FP64 takes 1 (LDS: P planes x CC x (TXY+P-1)^2 values).
 template <typename T, int P, int CC, int TXY>
 static void interp_cc(const InterpArgs<T> &a, hipStream_t st) {
+  constexpr int NT = 512;
   const int nt = a.K / TXY;
-  const int zc = a.K >= 64 ? a.K / 4 : a.K;
-  hipLaunchKernelGGL((k_interp<T, P, CC, TXY>), dim3(nt * nt, (a.K + zc - 1) / zc, a.C / CC),
-                     dim3(256), 0, st, a.boff, a.pbase, a.pw, a.perm, a.grid, a.comp_stride,
+  const int zc = a.K >= 64 ? a.K / 4 : a.K;  // <= 64
+  hipLaunchKernelGGL((k_interp<T, P, CC, TXY, NT>), dim3(nt * nt, (a.K + zc - 1) / zc, a.C / CC),
+                     dim3(NT), 0, st, a.boff, a.pbase, a.pw, a.perm, a.grid, a.comp_stride,
                      a.RS, a.K, a.C, zc, a.alpha, a.beta, a.out, a.ldout);
 }
 

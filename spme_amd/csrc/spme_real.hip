@@ -15,6 +15,7 @@
 // pairs the 1/r^3 terms of M_real and of the far-field subtraction cancel.
 #include <hip/hip_runtime.h>
 
+#include "spme_kernels.h"
 #include "spme_real.h"
 
 namespace spme {
@@ -80,7 +81,8 @@ __global__ __launch_bounds__(256) void k_cell_gather(const int *__restrict__ per
 // row, stored entry-major (entry k of row 64q + l at soff[q] + 64k + l) so
 // that a wave reads each entry array coalesced.  Each entry is a neighbour
 // j (cell order) and the 6 unique values of its symmetric 3x3 block
-// M_real + overlap, or j = -1 past the end of a row.
+// M_real + overlap; past the end of a row, the row's own index with a zero
+// block.
 
 __device__ __forceinline__ int cell_of(double p, double L, int nc) {
   double t = p / L;
@@ -176,7 +178,14 @@ __global__ __launch_bounds__(256) void k_pair_fill(const double *__restrict__ sp
     blk[5 * E + e] = T(F + g * ez * ez);
     ++k;
   });
-  for (; k < width[q]; ++k) col[base + 64L * k] = -1;
+  // pad the row with zero blocks on the row's own particle (no branch in the
+  // product loop)
+  for (; k < width[q]; ++k) {
+    const long e = base + 64L * k;
+    col[e] = s;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) blk[m * E + e] = T(0);
+  }
 }
 
 // One thread per row (cell order).  fs: forces in cell order [N][C].
@@ -190,7 +199,7 @@ __global__ __launch_bounds__(256) void k_real_spmv(const int *__restrict__ perm,
                                                    T self, const T *__restrict__ fs, int N,
                                                    T alpha, T beta, int accum,
                                                    T *__restrict__ out, long ldout) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= N) return;
   const int q = s >> 6, l = s & 63;
   T acc[C];
@@ -198,10 +207,10 @@ __global__ __launch_bounds__(256) void k_real_spmv(const int *__restrict__ perm,
   for (int c = 0; c < C; ++c) acc[c] = self * fs[(size_t)s * C + c];
   const long base = soff[q] + l;
   const int w = width[q];
+#pragma unroll 4
   for (int k = 0; k < w; ++k) {
     const long e = base + 64L * k;
     const int j = col[e];
-    if (j < 0) break;
     const T b0 = blk[e], b1 = blk[E + e], b2 = blk[2 * E + e];
     const T b3 = blk[3 * E + e], b4 = blk[4 * E + e], b5 = blk[5 * E + e];
     const T *fj = fs + (size_t)j * C;
