@@ -75,6 +75,25 @@ enum {
 int spme_engine_set_timing(spme_t *spme, int enable);
 int spme_engine_get_timing(spme_t *spme, double *ms_total, int *launches);
 
+/* Parameter-selection network (the reference has none; its cutoff2xi is
+ * declared only).  A 4 -> 64 -> 64 -> 4 tanh MLP in FP32; w holds
+ * W1[64][4], b1[64], W2[64][64], b2[64], W3[4][64], b3[4] (spme_mlp_nparam()
+ * floats).  All pointers are device memory; calls are asynchronous on stream
+ * (NULL: the default stream).
+ *   forward:  y[B][4] = net(x[B][4]); h1, h2 ([B][64] each, may be NULL)
+ *             receive the hidden activations for a backward pass.
+ *   backward: gradient of (1/B) sum_b sum_o mask[o] (y - t)^2 into
+ *             grad[nparam]; work holds B*(4 + 2*64) floats.
+ *   adam:     one Adam step (step counts from 1). */
+int spme_mlp_nparam(void);
+int spme_mlp_forward(const float *w, const float *x, int B, float *y, float *h1, float *h2,
+                     void *stream);
+int spme_mlp_backward(const float *w, const float *x, const float *y, const float *t,
+                      const float *mask, int B, const float *h1, const float *h2,
+                      float *work, float *grad, void *stream);
+int spme_mlp_adam(float *w, const float *g, float *m1, float *m2, float lr, float beta1,
+                  float beta2, float eps, int step, void *stream);
+
 /* Layout and sizes, for tests and benchmarks. */
 typedef struct spme_info {
   int K, P, N, precision, max_rhs;

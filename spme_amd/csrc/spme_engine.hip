@@ -108,13 +108,6 @@ void make_bsq(int K, int p, std::vector<double> &bsq) {
   }
 }
 
-// psi(k) of old-version/spme.c:43-60 with a_i = a_j = 1.
-double psi(double k, double xi) {
-  const double kx = k / xi, kx2 = kx * kx, k2 = k * k;
-  return 6.0 * M_PI * std::exp(-kx2 / 4.0) * (1.0 - k2 / 3.0) / k2 *
-         (1.0 + kx2 * (0.25 + kx2 / 8.0));
-}
-
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -172,23 +165,15 @@ int upload_tables(_spme_t *e) {
   std::vector<double> bsq;
   make_bsq(K, e->P, bsq);
   const double L = e->prm.box_size, V = L * L * L, xi = e->prm.xi;
-  const double dk = 2.0 * M_PI / L;
-  std::vector<T> tab((size_t)K * K * e->Khp, T(0));
-  for (int z = 0; z < K; ++z) {
-    const double kz = (z <= K / 2 ? z : z - K) * dk;
-    for (int y = 0; y < K; ++y) {
-      const double ky = (y <= K / 2 ? y : y - K) * dk;
-      for (int x = 0; x <= K / 2; ++x) {
-        const double kx = x * dk;
-        const double kk = std::sqrt(kx * kx + ky * ky + kz * kz);
-        double v = 0.0;
-        if (kk > 0.0) v = psi(kk, xi) * bsq[x] * bsq[y] * bsq[z] / V;
-        tab[((size_t)z * K + y) * e->Khp + x] = T(v);
-      }
-    }
-  }
-  HIP_TRY(hipMalloc(&e->tab, tab.size() * sizeof(T)));
-  HIP_TRY(hipMemcpy(e->tab, tab.data(), tab.size() * sizeof(T), hipMemcpyHostToDevice));
+  double *d_bsq = nullptr;
+  HIP_TRY(hipMalloc(&d_bsq, sizeof(double) * K));
+  HIP_TRY(hipMemcpy(d_bsq, bsq.data(), sizeof(double) * K, hipMemcpyHostToDevice));
+  HIP_TRY(hipMalloc(&e->tab, sizeof(T) * (size_t)K * K * e->Khp));
+  launch_influence_table<T>(d_bsq, K, e->Khp, 2.0 * M_PI / L, xi, 1.0 / V, (T *)e->tab,
+                            e->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  HIP_TRY(hipFree(d_bsq));
   std::vector<T> tw(2 * K);
   for (int t = 0; t < K; ++t) {
     const double a = -2.0 * M_PI * t / K;

@@ -765,7 +765,48 @@ __global__ __launch_bounds__(NT) void k_fft_z_infl(T *__restrict__ grid, long co
 }
 
 // ---------------------------------------------------------------------------
+// Influence table (old-version/spme.c:128-150), FP64:
+//   tab[(z*K + y)*Khp + x] = psi(|k|) |b_x|^2 |b_y|^2 |b_z|^2 / V, x <= K/2,
+// psi(k) = 6 pi e^{-k^2/4xi^2} (1 - k^2/3)(1 + k^2/4xi^2 + k^4/8xi^4)/k^2
+// (old-version/spme.c:43-60 with unit radii); 0 at k = 0 and in padding.
+template <typename T>
+__global__ __launch_bounds__(256) void k_influence_table(const double *__restrict__ bsq,
+                                                         int K, int Khp, double dk, double xi,
+                                                         double inv_v, T *__restrict__ tab) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)K * K * Khp) return;
+  const int x = int(t % Khp);
+  const long zy = t / Khp;
+  const int y = int(zy % K), z = int(zy / K);
+  double v = 0.0;
+  if (x <= K / 2) {
+    const double kx = x * dk, ky = (y <= K / 2 ? y : y - K) * dk;
+    const double kz = (z <= K / 2 ? z : z - K) * dk;
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    if (k2 > 0.0) {
+      const double q = k2 / (xi * xi);
+      const double psi = 6.0 * M_PI * exp(-q / 4.0) * (1.0 - k2 / 3.0) / k2 *
+                         (1.0 + q * (0.25 + q / 8.0));
+      v = psi * bsq[x] * bsq[y] * bsq[z] * inv_v;
+    }
+  }
+  tab[t] = T(v);
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers.
+
+template <typename T>
+void launch_influence_table(const double *bsq, int K, int Khp, double dk, double xi,
+                            double inv_v, T *tab, hipStream_t st) {
+  const long n = (long)K * K * Khp;
+  hipLaunchKernelGGL(k_influence_table<T>, dim3(int((n + 255) / 256)), dim3(256), 0, st, bsq, K,
+                     Khp, dk, xi, inv_v, tab);
+}
+template void launch_influence_table<float>(const double *, int, int, double, double, double,
+                                            float *, hipStream_t);
+template void launch_influence_table<double>(const double *, int, int, double, double, double,
+                                             double *, hipStream_t);
 
 static int grid1(long n, int b) { return int((n + b - 1) / b); }
 

@@ -62,5 +62,8 @@ template <typename T> int launch_interp(const InterpArgs<T> &a, hipStream_t st);
 template <typename T>
 int launch_fft_influence(const FftArgs<T> &a, hipStream_t st, hipEvent_t *ev = nullptr);
 bool fft_size_supported(int K);
+template <typename T>
+void launch_influence_table(const double *bsq, int K, int Khp, double dk, double xi,
+                            double inv_v, T *tab, hipStream_t st);
 
 }  // namespace spme

@@ -21,6 +21,7 @@ EXPORTED = [
     "spme_last_error", "spme_engine_create", "spme_engine_destroy",
     "spme_engine_set_positions", "spme_engine_apply", "spme_engine_info",
     "spme_engine_debug_stage", "spme_engine_set_timing", "spme_engine_get_timing",
+    "spme_mlp_nparam", "spme_mlp_forward", "spme_mlp_backward", "spme_mlp_adam",
 ]
 
 STAGES = ["permute", "spread", "fft_x_fwd", "fft_y_fwd", "fft_z_infl",
@@ -73,6 +74,11 @@ def load() -> ctypes.CDLL:
     L.spme_engine_debug_stage.argtypes = [vp, i, i, vp, l, vp]
     L.spme_engine_set_timing.argtypes = [vp, i]
     L.spme_engine_get_timing.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_int)]
+    f = ctypes.c_float
+    L.spme_mlp_nparam.restype = i
+    L.spme_mlp_forward.argtypes = [vp, vp, i, vp, vp, vp, vp]
+    L.spme_mlp_backward.argtypes = [vp, vp, vp, vp, vp, i, vp, vp, vp, vp, vp]
+    L.spme_mlp_adam.argtypes = [vp, vp, vp, vp, f, f, f, f, i, vp]
     _lib = L
     return L
 
