@@ -346,6 +346,7 @@ int build_P(_spme_t *e, hipStream_t st) {
 
 int set_positions(_spme_t *e, const double *pos, int on_device, hipStream_t st) {
   if (!pos) return fail(-1, "positions are NULL");
+  (void)hipGetLastError();
   HIP_TRY(hipMemcpyAsync(e->pos, pos, sizeof(double) * 3 * e->N,
                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
   int rc = e->prec == SPME_FP32 ? build_P<float>(e, st) : build_P<double>(e, st);
@@ -459,6 +460,9 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
 
 int apply(_spme_t *e, int nrhs, double alpha, const void *in, long ldin, double beta,
           void *out, long ldout, int parts, hipStream_t st, int stop_after = 0) {
+  // hipGetLastError is sticky per thread: clear what earlier, unrelated
+  // calls left so that the check after the launches sees only ours
+  (void)hipGetLastError();
   if (!e->have_pos) return fail(-1, "positions have not been set");
   if (nrhs < 1 || nrhs > e->max_rhs)
     return fail(-1, "nrhs must be in [1, %d] (got %d)", e->max_rhs, nrhs);

@@ -170,9 +170,11 @@ int spme_mlp_forward(const float *w, const float *x, int B, float *y, float *h1,
                      void *stream) {
   if (!w || !x || !y || B < 0) return -1;
   if (B == 0) return 0;
+  (void)hipGetLastError();  // clear an error left by an earlier, unrelated call
   hipLaunchKernelGGL(k_mlp_forward, dim3(grid1(B, 256)), dim3(256), 0, (hipStream_t)stream, w,
                      x, B, y, h1, h2);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : -100 - int(e);
 }
 
 int spme_mlp_backward(const float *w, const float *x, const float *y, const float *t,
@@ -180,20 +182,24 @@ int spme_mlp_backward(const float *w, const float *x, const float *y, const floa
                       float *work, float *grad, void *stream) {
   if (!w || !x || !y || !t || !mask || !h1 || !h2 || !work || !grad || B <= 0) return -1;
   hipStream_t st = (hipStream_t)stream;
+  (void)hipGetLastError();
   float *dy = work, *d1 = work + (size_t)B * NO, *d2 = d1 + (size_t)B * NH;
   hipLaunchKernelGGL(k_mlp_delta, dim3(grid1(B, 256)), dim3(256), 0, st, w, y, t, mask, B, h1,
                      h2, dy, d1, d2);
   hipLaunchKernelGGL(k_mlp_grad, dim3(grid1(NPARAM, 256)), dim3(256), 0, st, x, B, h1, h2, dy,
                      d1, d2, grad);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : -100 - int(e);
 }
 
 int spme_mlp_adam(float *w, const float *g, float *m1, float *m2, float lr, float beta1,
                   float beta2, float eps, int step, void *stream) {
   if (!w || !g || !m1 || !m2 || step < 1) return -1;
+  (void)hipGetLastError();
   hipLaunchKernelGGL(k_mlp_adam, dim3(grid1(NPARAM, 256)), dim3(256), 0, (hipStream_t)stream,
                      w, g, m1, m2, lr, beta1, beta2, eps, step);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : -100 - int(e);
 }
 
 int spme_mlp_nparam(void) { return NPARAM; }

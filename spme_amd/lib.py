@@ -55,6 +55,14 @@ def load() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
         return _lib
+    # torch bundles its own HIP runtime with the same soname
+    # (libamdhip64.so.7) as the one this library links; the process keeps
+    # whichever is loaded first.  Load torch's first so that one runtime
+    # serves both and torch's streams and buffers are valid here.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise SpmeError(f"{LIB_PATH} is missing: build it with "
                         "`python -m spme_amd.build` (no CPU fallback exists)")

@@ -114,6 +114,15 @@ class Net:
         return hist
 
     # -- selection -------------------------------------------------------
+    def in_domain(self, n, phi, tol):
+        """True inside the (N, phi, tol) box the network was trained on; the
+        network is not trusted to extrapolate outside it."""
+        d = getattr(self, "meta", {}).get("domain")
+        if not d:
+            return True
+        return (d["n"][0] <= n <= d["n"][1] and d["phi"][0] <= phi <= d["phi"][1]
+                and d["tol"][0] <= tol <= d["tol"][1])
+
     def predict(self, n, phi, tol):
         xs = (features(n, phi, tol) - self.x_mean) / self.x_std
         ys = self.forward_raw(xs[None, :])[0] * self.y_std + self.y_mean
@@ -131,7 +140,7 @@ class Net:
         while 4 * p >= K and p > 4:
             p -= 2
         ok = (0.05 < xi < 3.0) and (0.5 < rc <= L / 3.0) and 4 * p < K
-        if not ok:
+        if not ok or not self.in_domain(n, phi, tol):
             a = PR.analytic_select(n, phi, tol, nrhs=nrhs, k_max=k_max)
             return a.xi, a.K, a.p, a.rc, "analytic"
         return xi, K, p, rc, "learned"
@@ -150,17 +159,23 @@ class Net:
         net.meta = json.loads(str(z["meta"]))
         return net
 
+    def set_meta(self, **kw):
+        self.meta = dict(getattr(self, "meta", {}), **kw)
 
-def labels_from_runs(runs, tols=(1e-3, 3e-4, 1e-4, 3e-5, 1e-5, 3e-6, 1e-6, 3e-7, 1e-7)):
+
+TOLS = (1e-3, 3e-4, 1e-4, 3e-5, 1e-5, 3e-6, 1e-6, 3e-7, 1e-7)
+
+
+def labels_from_runs(runs, tols=TOLS, margin=1.0):
     """For every system and tolerance: the fastest run whose measured error is
-    within the tolerance.  runs: dict of equal-length arrays with keys
+    within tol / margin.  runs: dict of equal-length arrays with keys
     sys, n, phi, xi, K, p, rc, err, time.  Returns (X, T, info)."""
     X, T, info = [], [], []
     sysids = np.unique(runs["sys"])
     for s in sysids:
         sel = np.where(runs["sys"] == s)[0]
         for tol in tols:
-            ok = sel[runs["err"][sel] <= tol]
+            ok = sel[runs["err"][sel] <= tol / margin]
             if ok.size == 0:
                 continue
             b = ok[np.argmin(runs["time"][ok])]
@@ -170,15 +185,25 @@ def labels_from_runs(runs, tols=(1e-3, 3e-4, 1e-4, 3e-5, 1e-5, 3e-6, 1e-6, 3e-7,
     return np.array(X), np.array(T), info
 
 
-def fit(runs, epochs=4000, lr=3e-3, seed=0, log_every=500):
-    """Train a network on measured runs; returns (net, history)."""
-    X, T, _ = labels_from_runs(runs)
+def training_domain(runs, tols=TOLS):
+    """The (N, phi, tol) box covered by the training runs."""
+    return {"n": [int(runs["n"].min()), int(runs["n"].max())],
+            "phi": [float(runs["phi"].min()), float(runs["phi"].max())],
+            "tol": [float(min(tols)), float(max(tols))]}
+
+
+def fit(runs, epochs=4000, lr=3e-3, seed=0, log_every=500, margin=1.5):
+    """Train a network on measured runs; returns (net, history).  Labels keep
+    a margin below the tolerance so that predictions, which interpolate
+    between neighbouring systems, still land inside it."""
+    X, T, _ = labels_from_runs(runs, margin=margin)
     xm, xsd = X.mean(0), X.std(0) + 1e-12
     ym, ysd = T.mean(0), T.std(0) + 1e-12
     net = Net(init_params(seed), xm, xsd, ym, ysd)
     hist = net.train((X - xm) / xsd, (T - ym) / ysd, epochs=epochs, lr=lr,
                      log_every=log_every)
     net.meta = {"samples": int(len(X)), "epochs": epochs, "lr": lr, "seed": seed,
+                "margin": margin, "domain": training_domain(runs),
                 "final_loss": hist[-1][1] if hist else None}
     return net, hist
 
@@ -191,12 +216,19 @@ def select(n, phi, tol, nrhs=1, k_max=256):
     return a.xi, a.K, a.p, a.rc, "analytic"
 
 
+def load_runs(*paths):
+    """Concatenate run files written by the training-data generator."""
+    parts = [np.load(p) for p in paths]
+    return {k: np.concatenate([z[k] for z in parts]) for k in parts[0].files}
+
+
 if __name__ == "__main__":
+    # python -m spme_amd.mlp OUT.npz RUNS.npz [RUNS.npz ...]
     import sys
-    data = np.load(sys.argv[1])
-    runs = {k: data[k] for k in data.files}
+    runs = load_runs(*sys.argv[2:])
     net, hist = fit(runs)
     for ep, loss in hist:
-        print(f"epoch {ep} loss {loss:.4e}")
-    net.save(meta=net.meta)
-    print("saved", WEIGHTS)
+        print(f"epoch {ep} loss {loss:.4e}", flush=True)
+    net.meta["runs"] = int(len(runs["err"]))
+    net.save(sys.argv[1], meta=net.meta)
+    print("saved", sys.argv[1], net.meta, flush=True)

@@ -1,0 +1,85 @@
+"""What the learned parameter selector gains over the analytic estimate.
+
+Held-out systems (tests/golden/mlp_eval.npz: seeds not used in training,
+plus N = 500,000 at phi = 0.3, outside the training range) with FP64
+direct-Ewald rows.  For each system and target tolerance three selections
+pick (xi, K, p, rc):
+  analytic       the error model at the requested tolerance,
+  analytic/4     the error model asked for tol/4 (the safety factor the
+                 model needs to meet every held-out target),
+  learned        the network (analytic outside its training domain).
+Each is run as the full M.f in FP64 with one right-hand side, the form the
+training labels were measured in: the relative error against the direct rows
+and the apply time.  The test checks that the learned selector meets the
+tolerance at least as often as the plain analytic one, and that at equal
+reliability (against analytic/4) it is faster overall.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from spme_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOLS = (1e-4, 1e-6)
+SAFETY = 4.0
+
+
+def _run(torch, E, sy, f_d, rows, direct, prm):
+    xi, K, p, rc = prm
+    n = sy.pos.shape[0]
+    eng = E.Engine(xi, K, p, n, sy.L, rc=rc, precision=E.SPME_FP64, max_rhs=1)
+    eng.set_positions(sy.pos)
+    u = eng.apply(f_d, parts=E.SPME_FULL)
+    mine = u[0].cpu().numpy().reshape(n, 3)[rows].reshape(-1)
+    err = float(np.linalg.norm(mine - direct) / np.linalg.norm(direct))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(3):
+        eng.apply(f_d, u, parts=E.SPME_FULL)
+    e1.record()
+    torch.cuda.synchronize()
+    eng.close()
+    return err, e0.elapsed_time(e1) / 3
+
+
+def test_learned_selector_gain():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from spme_amd import engine as E
+    from spme_amd import mlp
+    from spme_amd import params as PR
+    if not os.path.exists(mlp.WEIGHTS):
+        pytest.skip("no trained weights")
+    net = mlp.Net.load()
+    ev = np.load(os.path.join(HERE, "golden", "mlp_eval.npz"))
+    rows_all, direct_all = ev["rows"], ev["direct"]
+    res = {"analytic": [], "analytic/4": [], "learned": []}
+    for k in range(len(ev["seed"])):
+        seed, n, phi = int(ev["seed"][k]), int(ev["n"][k]), float(ev["phi"][k])
+        sy = W.make_system(seed, n, phi)
+        f_d = torch.tensor(W.forces(seed, n, 1), device="cuda")
+        for tol in TOLS:
+            picks = {"analytic": PR.analytic_select(n, phi, tol).as_tuple(),
+                     "analytic/4": PR.analytic_select(n, phi, tol / SAFETY).as_tuple()}
+            xi, K, p, rc, how = net.select(n, phi, tol)
+            picks["learned"] = (xi, K, p, rc)
+            line = f"N={n:7d} phi={phi:.3f} tol={tol:.0e}"
+            for name, prm in picks.items():
+                err, ms = _run(torch, E, sy, f_d, rows_all[k], direct_all[k], prm)
+                res[name].append((err <= tol, ms))
+                tag = f"{name}({how})" if name == "learned" else name
+                line += (f" | {tag} K={prm[1]} p={prm[2]} xi={prm[0]:.3f} rc={prm[3]:.2f}"
+                         f" err={err:.2e} {ms:.3f}ms")
+            print(line, flush=True)
+    summ = {k: (sum(ok for ok, _ in v), sum(ms for _, ms in v)) for k, v in res.items()}
+    total = len(res["learned"])
+    for k, (ok, ms) in summ.items():
+        print(f"{k:11s} meets tolerance {ok}/{total}, total time {ms:.2f} ms", flush=True)
+    assert summ["learned"][0] >= summ["analytic"][0]
+    assert summ["learned"][0] >= summ["analytic/4"][0] - 1
+    assert summ["learned"][1] < summ["analytic/4"][1]
