@@ -75,6 +75,62 @@ enum {
 int spme_engine_set_timing(spme_t *spme, int enable);
 int spme_engine_get_timing(spme_t *spme, double *ms_total, int *launches);
 
+/* Tile shapes of the spreading, interpolation and real-space kernels (the
+ * reference has no counterpart; its kernels have fixed shapes).  A field of
+ * 0 selects the default.  No setting changes the arithmetic: every valid
+ * choice gives bitwise the same result, so tuning is safe at any time.
+ *   spread_cc   components per spreading pass: 3, 6 or 12; ignored when it
+ *               does not divide 3*nrhs
+ *   spread_zc   z planes per spreading block, 1..256 (clamped to K)
+ *   interp_zc   z planes per interpolation block, 1..64 (clamped to K)
+ *   *_persist   grouped launches only (spme_group_*): resident blocks per
+ *               compute unit of a persistent launch of that stage, 1..64;
+ *               0 launches one block per tile. */
+typedef struct spme_tiles {
+  int spread_cc;
+  int spread_zc;
+  int interp_zc;
+  int spread_persist;
+  int interp_persist;
+  int real_persist;
+} spme_tiles_t;
+
+/* NULL restores the defaults. */
+int spme_engine_set_tiles(spme_t *spme, const spme_tiles_t *tiles);
+/* The shapes an apply of nrhs right-hand sides uses, defaults resolved. */
+int spme_engine_get_tiles(const spme_t *spme, int nrhs, spme_tiles_t *tiles);
+/* Autotuner: times every candidate shape of the spreading and interpolation
+ * kernels on the engine's current positions for batches of nrhs (reps timed
+ * launches each), keeps the fastest (as set_tiles) and returns it in best
+ * (may be NULL).  Synchronous on stream; overwrites the engine's mesh. */
+int spme_engine_autotune(spme_t *spme, int nrhs, int reps, spme_tiles_t *best, void *stream);
+
+/* Groups of independent systems (uneven N, box and mesh) applied together:
+ * every stage of the apply is one launch over the tiles of all systems, and
+ * with graph != 0 the whole apply is one captured hipGraph, replayed on
+ * later calls.  The engines must have positions set, one precision and one
+ * B-spline order, and max_rhs >= nrhs; they stay owned by the caller and
+ * must outlive the group.  A group gives bitwise the result of the separate
+ * spme_engine_apply calls.  It uses the engines' meshes and buffers: do not
+ * apply a member engine on another stream while a group apply may run.
+ * Changing an engine's positions is allowed (the group notices). */
+typedef struct _spme_group_t spme_group_t;
+int spme_group_create(int nsys, spme_t *const *engines, int nrhs, spme_group_t **out);
+int spme_group_destroy(spme_group_t *group);
+/* Shapes for every member (zc values clamp per system); NULL: defaults. */
+int spme_group_set_tiles(spme_group_t *group, const spme_tiles_t *tiles);
+int spme_group_get_tiles(const spme_group_t *group, spme_tiles_t *tiles);
+/* Autotuner for the grouped kernels (as spme_engine_autotune, over the
+ * group's systems; the real-space stage is tuned when parts has it). */
+int spme_group_autotune(spme_group_t *group, int parts, int reps, spme_tiles_t *best,
+                        void *stream);
+/* For s < nsys: out[s][r*ldout[s] + 3i + c] <- beta*out + alpha*(M_s f_{s,r})
+ * for r < nrhs, in[s] and out[s] device pointers of the engines' precision.
+ * Asynchronous on stream (NULL: the default stream). */
+int spme_group_apply(spme_group_t *group, double alpha, const void *const *in,
+                     const long *ldin, double beta, void *const *out, const long *ldout,
+                     int parts, int graph, void *stream);
+
 /* Parameter-selection network (the reference has none; its cutoff2xi is
  * declared only).  A 4 -> 64 -> 64 -> 4 tanh MLP in FP32; w holds
  * W1[64][4], b1[64], W2[64][64], b2[64], W3[4][64], b3[4] (spme_mlp_nparam()

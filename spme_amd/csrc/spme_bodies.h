@@ -1,0 +1,684 @@
+// Device bodies of the apply-path kernels.  Each body processes one logical
+// block (one row or element for the elementwise ones) given explicitly, so
+// that a kernel can run it once per hardware block for one system
+// (spme_kernels.hip) or loop it over the blocks of many systems
+// (spme_group.hip).  Included by those sources only.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "spme_kernels.h"
+
+namespace spme {
+
+// fs[s*C + 3r + d] = in[r*ldin + 3*perm[s] + d]  (sorted, particle-major);
+// element t of the N*C outputs.
+template <typename T>
+__device__ __forceinline__ void permute_elem(const int *__restrict__ perm,
+                                             const T *__restrict__ in, long ldin,
+                                             T *__restrict__ fs, int C, long t) {
+  const int s = int(t / C), c = int(t - (long)s * C);
+  const int r = c / 3, d = c - 3 * r;
+  fs[t] = in[r * ldin + 3L * perm[s] + d];
+}
+
+// ---------------------------------------------------------------------------
+// Spread (old-version/spme_kernel.c:231-312): grid[c] = sum_i q_i (x) f_ic.
+// One wave owns a TX x TY column of cells over a z range and keeps a window
+// of P planes per component in registers; particles are visited in
+// increasing base z, and the lowest plane of the window is final (and
+// stored) once the sweep has passed every base that reaches it; the window
+// then shifts by one plane.  Every cell is written exactly once, so the
+// mesh needs no clearing and no atomics are used.
+// Latency: the bucket offsets of the whole sweep are read into LDS up front,
+// and each step's particles (base, weights, forces) are staged into LDS with
+// one batch of independent loads.
+template <typename T, int P, int CC, int TX, int TY>
+__device__ __forceinline__ void spread_block(const int *__restrict__ boff,
+                                             const int *__restrict__ pbase,
+                                             const T *__restrict__ pw, const T *__restrict__ fs,
+                                             int C, T *__restrict__ grid, long comp_stride,
+                                             int RS, int K, int ZC, int lb) {
+  static_assert(TX * TY == 64, "one wave per tile");
+  constexpr int NXR = 1 + (P + TX - 2) / TX;
+  constexpr int NYR = 1 + (P + TY - 2) / TY;
+  constexpr int NTL = NXR * NYR;
+  constexpr int MAXST = 256 + P;  // >= ZC + P - 1
+  constexpr int MAXB = 16;        // particles staged per batch
+  constexpr int NW = 3 * P;
+  __shared__ int s_lo[NTL * MAXST];
+  __shared__ int s_n[NTL * MAXST];
+  __shared__ int s_pb[MAXB];
+  __shared__ T s_w[MAXB * NW];
+  __shared__ T s_f[MAXB * CC];
+  const int ntx = K / TX, nty = K / TY, nz = (K + ZC - 1) / ZC;
+  const int tile = lb % (ntx * nty), rest = lb / (ntx * nty);
+  const int tx = tile % ntx, ty = tile / ntx;
+  const int lane = threadIdx.x;
+  const int x = tx * TX + lane % TX, y = ty * TY + lane / TX;
+  const int z0 = (rest % nz) * ZC, zend = min(z0 + ZC, K);
+  const int cc0 = (rest / nz) * CC;
+  const int zs = z0 - (P - 1);
+  const int nsteps = zend - zs;
+
+  for (int q = lane; q < NTL * nsteps; q += 64) {
+    const int t = q / nsteps, st = q - t * nsteps;
+    int tyy = ty - t / NXR, txx = tx - t % NXR;
+    tyy += tyy < 0 ? nty : 0;
+    txx += txx < 0 ? ntx : 0;
+    int bz = zs + st;
+    bz += bz < 0 ? K : 0;
+    const int key = (tyy * ntx + txx) * K + bz;
+    const int lo = boff[key];
+    s_lo[q] = lo;
+    s_n[q] = boff[key + 1] - lo;
+  }
+  __syncthreads();
+
+  T acc[P][CC];
+#pragma unroll
+  for (int j = 0; j < P; ++j)
+#pragma unroll
+    for (int c = 0; c < CC; ++c) acc[j][c] = T(0);
+
+  for (int st = 0; st < nsteps; ++st) {
+    int cnt[NTL], lo[NTL];
+    int n = 0;
+#pragma unroll
+    for (int t = 0; t < NTL; ++t) {
+      cnt[t] = s_n[t * nsteps + st];
+      lo[t] = s_lo[t * nsteps + st];
+      n += cnt[t];
+    }
+    // sorted index of the k-th particle of this step
+    auto locate = [&](int k) {
+      int i = lo[0] + k;
+#pragma unroll
+      for (int t = 0; t < NTL - 1; ++t) {
+        if (k >= cnt[t]) {
+          k -= cnt[t];
+          i = lo[t + 1] + k;
+        } else {
+          break;
+        }
+      }
+      return i;
+    };
+    for (int b0 = 0; b0 < n; b0 += MAXB) {
+      const int nb = min(MAXB, n - b0);
+      // stage the batch: all loads first, then the LDS stores
+      constexpr int RW = (MAXB * NW + 63) / 64, RF = (MAXB * CC + 63) / 64;
+      int vpb = 0;
+      T vw[RW], vf[RF];
+      if (lane < nb) vpb = pbase[locate(b0 + lane)];
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        const int g = lane + 64 * r;
+        vw[r] = g < nb * NW ? pw[(size_t)locate(b0 + g / NW) * NW + g % NW] : T(0);
+      }
+#pragma unroll
+      for (int r = 0; r < RF; ++r) {
+        const int g = lane + 64 * r;
+        vf[r] = g < nb * CC ? fs[(size_t)locate(b0 + g / CC) * C + cc0 + g % CC] : T(0);
+      }
+      if (lane < nb) s_pb[lane] = vpb;
+#pragma unroll
+      for (int r = 0; r < RW; ++r)
+        if (lane + 64 * r < MAXB * NW) s_w[lane + 64 * r] = vw[r];
+#pragma unroll
+      for (int r = 0; r < RF; ++r)
+        if (lane + 64 * r < MAXB * CC) s_f[lane + 64 * r] = vf[r];
+      __syncthreads();
+      for (int k = 0; k < nb; ++k) {
+        const int pb = s_pb[k];
+        int dx = x - (pb & 1023);
+        dx += dx < 0 ? K : 0;
+        int dy = y - ((pb >> 10) & 1023);
+        dy += dy < 0 ? K : 0;
+        if (dx < P && dy < P) {
+          const T *w = s_w + k * NW;
+          const T wxy = w[dx] * w[P + dy];
+          T wf[CC];
+#pragma unroll
+          for (int c = 0; c < CC; ++c) wf[c] = wxy * s_f[k * CC + c];
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const T wz = w[2 * P + j];
+#pragma unroll
+            for (int c = 0; c < CC; ++c) acc[j][c] += wz * wf[c];
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (st >= P - 1) {
+      T *g = grid + ((size_t)(zs + st) * K + y) * RS + x + (size_t)cc0 * comp_stride;
+#pragma unroll
+      for (int c = 0; c < CC; ++c) g[(size_t)c * comp_stride] = acc[0][c];
+    }
+#pragma unroll
+    for (int j = 0; j < P - 1; ++j)
+#pragma unroll
+      for (int c = 0; c < CC; ++c) acc[j][c] = acc[j + 1][c];
+#pragma unroll
+    for (int c = 0; c < CC; ++c) acc[P - 1][c] = T(0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Interpolation (old-version/spme_kernel.c:8-70):
+//   out[r*ldout + 3*i + d] = beta*out + alpha * sum_j q_i(j) grid[3r+d](j).
+// A block owns a TXY x TXY tile of stencil bases over a z range and a chunk
+// of CC components.  It streams the tile plus its (P-1)-wide halo through
+// LDS one z plane at a time (P+1 slots: the P planes the current particles
+// need and the one being filled); each particle is handled by exactly one
+// wave (lanes = the P x P stencil columns) and written without atomics.
+// The loads of step s+1 (its new plane and its particles) are issued before
+// step s is processed, so each step costs one barrier and no exposed
+// global-memory latency.
+template <typename T, int P, int CC, int TXY, int NT>
+__device__ __forceinline__ void interp_block(const int *__restrict__ boff,
+                                             const int *__restrict__ pbase,
+                                             const T *__restrict__ pw,
+                                             const int *__restrict__ perm,
+                                             const T *__restrict__ grid, long comp_stride,
+                                             int RS, int K, int C, int ZC, T alpha, T beta,
+                                             T *__restrict__ out, long ldout, int lb) {
+  constexpr int E = TXY + P - 1;
+  constexpr int EX = E | 1;  // odd row pitch
+  constexpr int PLANE = CC * E * EX;
+  constexpr int NSLOT = P + 1;
+  constexpr int NSUB = TXY / 8;  // sort tiles are 8 x 8
+  constexpr int NS2 = NSUB * NSUB;
+  constexpr int MAXST = 64;  // >= ZC
+  constexpr int NWV = NT / 64;
+  constexpr int MAXB = 32;  // particles staged per batch
+  constexpr int NW = 3 * P;
+  constexpr int NPL = (CC * E * E + NT - 1) / NT;  // plane values per thread
+  constexpr int RW = (MAXB * NW + NT - 1) / NT;    // weights per thread
+  __shared__ T win[NSLOT * PLANE];
+  __shared__ int s_lo[NS2 * MAXST];
+  __shared__ int s_n[NS2 * MAXST];
+  __shared__ int s_pb[2][MAXB], s_perm[2][MAXB];
+  __shared__ T s_w[2][MAXB * NW];
+  const int nt = K / TXY, nz = (K + ZC - 1) / ZC;
+  // logical block: tiles fastest, so each XCD gets a band of adjacent tiles
+  const int tile = lb % (nt * nt), rest = lb / (nt * nt);
+  const int tx = tile % nt, ty = tile / nt;
+  const int x0 = tx * TXY, y0 = ty * TXY;
+  const int z0 = (rest % nz) * ZC, zend = min(z0 + ZC, K);
+  const int nst = zend - z0;
+  const int cc0 = (rest / nz) * CC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const T *g0 = grid + (size_t)cc0 * comp_stride;
+
+  for (int q = tid; q < NS2 * nst; q += NT) {
+    const int sq = q / nst, st = q - sq * nst;
+    const int sx = tx * NSUB + sq % NSUB, sy = ty * NSUB + sq / NSUB;
+    const int key = (sy * (K / 8) + sx) * K + z0 + st;
+    const int lo = boff[key];
+    s_lo[q] = lo;
+    s_n[q] = boff[key + 1] - lo;
+  }
+  // per-thread plane offsets, fixed over the sweep: source offset within a
+  // mesh z plane (components of the chunk included) and LDS offset within a
+  // window slot; -1 past the plane
+  long src_off[NPL];
+  int dst_off[NPL];
+#pragma unroll
+  for (int r = 0; r < NPL; ++r) {
+    const int q = tid + NT * r;
+    const int c = q / (E * E), rr = q - c * (E * E);
+    const int ey = rr / E, ex = rr - ey * E;
+    int yy = y0 + ey, xx = x0 + ex;
+    yy -= yy >= K ? K : 0;
+    xx -= xx >= K ? K : 0;
+    src_off[r] = (long)c * comp_stride + (long)yy * RS + xx;
+    dst_off[r] = q < CC * E * E ? (c * E + ey) * EX + ex : -1;
+  }
+  const long zstride = (long)K * RS;
+  for (int j = 0; j < P - 1; ++j) {
+    const T *gz = g0 + ((z0 + j) % K) * zstride;
+    T *wz = win + ((z0 + j) % NSLOT) * PLANE;
+#pragma unroll
+    for (int r = 0; r < NPL; ++r)
+      if (dst_off[r] >= 0) wz[dst_off[r]] = gz[src_off[r]];
+  }
+  __syncthreads();
+
+  // registers carrying the next step's loads
+  T vp[NPL], vw[RW];
+  int vpb = 0, vpm = 0, nb_next = 0;
+  // issue the loads of step st, particles [b0, b0 + MAXB) of it
+  auto fetch = [&](int st, int b0, bool with_plane) {
+    int cnt[NS2], lo[NS2];
+    int n = 0;
+#pragma unroll
+    for (int q = 0; q < NS2; ++q) {
+      cnt[q] = s_n[q * nst + st];
+      lo[q] = s_lo[q * nst + st];
+      n += cnt[q];
+    }
+    auto locate = [&](int k) {
+      int i = lo[0] + k;
+#pragma unroll
+      for (int q = 0; q < NS2 - 1; ++q) {
+        if (k >= cnt[q]) {
+          k -= cnt[q];
+          i = lo[q + 1] + k;
+        } else {
+          break;
+        }
+      }
+      return i;
+    };
+    const int nb = max(0, min(MAXB, n - b0));
+    if (with_plane) {
+      const T *gz = g0 + ((z0 + st + P - 1) % K) * zstride;
+#pragma unroll
+      for (int r = 0; r < NPL; ++r) vp[r] = dst_off[r] >= 0 ? gz[src_off[r]] : T(0);
+    }
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int g = tid + NT * r;
+      vw[r] = g < nb * NW ? pw[(size_t)locate(b0 + g / NW) * NW + g % NW] : T(0);
+    }
+    if (tid < nb) {
+      const int i = locate(b0 + tid);
+      vpb = pbase[i];
+      vpm = perm[i];
+    }
+    nb_next = nb;
+    return n;
+  };
+  auto commit = [&](int st, int buf, bool with_plane) {
+    if (with_plane) {
+      T *wz = win + ((z0 + st + P - 1) % NSLOT) * PLANE;
+#pragma unroll
+      for (int r = 0; r < NPL; ++r)
+        if (dst_off[r] >= 0) wz[dst_off[r]] = vp[r];
+    }
+#pragma unroll
+    for (int r = 0; r < RW; ++r)
+      if (tid + NT * r < MAXB * NW) s_w[buf][tid + NT * r] = vw[r];
+    if (tid < MAXB) {
+      s_pb[buf][tid] = vpb;
+      s_perm[buf][tid] = vpm;
+    }
+  };
+
+  const int dx = lane % P, dy = lane / P;
+  const bool act = lane < P * P;
+  int n_cur = fetch(0, 0, true);
+  int b_cur = 0, st = 0, buf = 0;
+  while (st < nst) {
+    const int bz = z0 + st;
+    const int slot0 = bz % NSLOT;
+    const int nb = nb_next;
+    commit(st, buf, b_cur == 0);
+    __syncthreads();
+    // next unit of work: the rest of this step, or the next step
+    const int b_nxt = b_cur + MAXB < n_cur ? b_cur + MAXB : 0;
+    const int st_nxt = b_nxt ? st : st + 1;
+    int n_nxt = n_cur;
+    if (st_nxt < nst) n_nxt = fetch(st_nxt, b_nxt, b_nxt == 0);
+    for (int k = wave; k < nb; k += NWV) {
+      const int pb = s_pb[buf][k];
+      const int rx = (pb & 1023) - x0, ry = ((pb >> 10) & 1023) - y0;
+      const T *w = s_w[buf] + k * NW;
+      const T wxy = act ? w[dx] * w[P + dy] : T(0);
+      T acc[CC];
+#pragma unroll
+      for (int c = 0; c < CC; ++c) acc[c] = T(0);
+      if (act) {
+        const int off = (ry + dy) * EX + rx + dx;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          const T wz = w[2 * P + j];
+          const int sl = slot0 + j >= NSLOT ? slot0 + j - NSLOT : slot0 + j;
+          const T *pl = win + sl * PLANE + off;
+#pragma unroll
+          for (int c = 0; c < CC; ++c) acc[c] += wz * pl[c * E * EX];
+        }
+      }
+      const int o = s_perm[buf][k];
+#pragma unroll
+      for (int c = 0; c < CC; ++c) {
+        const T v = wave_sum(acc[c] * wxy);
+        if (lane == 0) {
+          const int comp = cc0 + c, r = comp / 3, d = comp - 3 * r;
+          T *u = out + r * ldout + 3L * o + d;
+          *u = beta == T(0) ? alpha * v : beta * *u + alpha * v;
+        }
+      }
+    }
+    b_cur = b_nxt;
+    st = st_nxt;
+    n_cur = n_nxt;
+    buf ^= 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FFT passes: four-step transforms of length K = N1 * N2 (Split<K>).  Step A:
+// per n2, an N1-point FFT over x[n1 N2 + n2] in registers, times
+// W_K^{n2 k1}, stored at k1 N2 + n2.  Step B: per k1, an N2-point FFT over
+// that contiguous block in registers gives X[k1 + N1 k2].  One LDS transpose
+// per transform.  tw[t] = exp(-2 pi i t/K), t < K.
+
+template <typename T, int K>
+__device__ __forceinline__ void load_tw(cpx<T> *s_tw, const cpx<T> *tw) {
+  for (int t = threadIdx.x; t < K; t += blockDim.x) s_tw[t] = tw[t];
+}
+
+// Forward r2c along x: two real rows per complex line (z = a + i b).
+template <typename T, int K, int NL, int NT>
+__device__ __forceinline__ void fft_x_r2c_block(T *__restrict__ grid, int RS, long npairs,
+                                                const cpx<T> *__restrict__ tw, int lb,
+                                                cpx<T> *buf, cpx<T> *s_tw) {
+  constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
+  constexpr int Kh = K / 2 + 1;
+  constexpr int LS = K + 1;
+  load_tw<T, K>(s_tw, tw);
+  const long q0 = (long)lb * NL;
+  for (int t = threadIdx.x; t < NL * K; t += NT) {
+    const int l = t / K, n = t - l * K;
+    const long q = q0 + l;
+    cpx<T> v = {T(0), T(0)};
+    if (q < npairs) {
+      const T *ra = grid + (2 * q) * RS;
+      v = {ra[n], ra[RS + n]};
+    }
+    buf[l * LS + n] = v;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NL * N2; t += NT) {  // step A, in place per thread
+    const int l = t / N2, n2 = t - l * N2;
+    cpx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) a[n1] = buf[l * LS + n1 * N2 + n2];
+    reg_fft<N1, -1>(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<-1>(s_tw, n2 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) buf[l * LS + k1 * N2 + n2] = a[k1];
+  }
+  __syncthreads();
+  constexpr int PB = (NL * N1 + NT - 1) / NT;
+  cpx<T> b[PB][N2];
+#pragma unroll
+  for (int u = 0; u < PB; ++u) {  // step B into registers
+    const int t = threadIdx.x + u * NT;
+    if (t < NL * N1) {
+      const int l = t / N1, k1 = t - l * N1;
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) b[u][n2] = buf[l * LS + k1 * N2 + n2];
+      reg_fft<N2, -1>(b[u]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PB; ++u) {  // natural order back into LDS
+    const int t = threadIdx.x + u * NT;
+    if (t < NL * N1) {
+      const int l = t / N1, k1 = t - l * N1;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) buf[l * LS + k1 + N1 * k2] = b[u][k2];
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NL * Kh; t += NT) {
+    const int l = t / Kh, k = t - l * Kh;
+    const long q = q0 + l;
+    if (q < npairs) {
+      const cpx<T> Z = buf[l * LS + k];
+      const cpx<T> Zm = cconj(buf[l * LS + (k == 0 ? 0 : K - k)]);
+      const cpx<T> A = {T(0.5) * (Z.x + Zm.x), T(0.5) * (Z.y + Zm.y)};
+      const cpx<T> D = csub(Z, Zm);
+      const cpx<T> B = {T(0.5) * D.y, T(-0.5) * D.x};
+      cpx<T> *ca = reinterpret_cast<cpx<T> *>(grid + (2 * q) * RS);
+      ca[k] = A;
+      ca[RS / 2 + k] = B;
+    }
+  }
+}
+
+// Backward c2r along x.  The imaginary parts of the k = 0 and k = K/2
+// entries are dropped, so the result is the real part of the inverse DFT.
+template <typename T, int K, int NL, int NT>
+__device__ __forceinline__ void fft_x_c2r_block(T *__restrict__ grid, int RS, long npairs,
+                                                const cpx<T> *__restrict__ tw, int lb,
+                                                cpx<T> *buf, cpx<T> *s_tw) {
+  constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
+  constexpr int Kh = K / 2 + 1;
+  constexpr int LS = K + 1;
+  load_tw<T, K>(s_tw, tw);
+  const long q0 = (long)lb * NL;
+  for (int t = threadIdx.x; t < NL * Kh; t += NT) {
+    const int l = t / Kh, k = t - l * Kh;
+    const long q = q0 + l;
+    cpx<T> A = {T(0), T(0)}, B = {T(0), T(0)};
+    if (q < npairs) {
+      const cpx<T> *ca = reinterpret_cast<const cpx<T> *>(grid + (2 * q) * RS);
+      A = ca[k];
+      B = ca[RS / 2 + k];
+    }
+    if (k == 0 || k == K / 2) {
+      A.y = T(0);
+      B.y = T(0);
+    }
+    buf[l * LS + k] = {A.x - B.y, A.y + B.x};
+    if (k > 0 && k < K / 2) buf[l * LS + K - k] = {A.x + B.y, B.x - A.y};
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NL * N2; t += NT) {  // step A, in place per thread
+    const int l = t / N2, n2 = t - l * N2;
+    cpx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1) a[n1] = buf[l * LS + n1 * N2 + n2];
+    reg_fft<N1, +1>(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<+1>(s_tw, n2 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) buf[l * LS + k1 * N2 + n2] = a[k1];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NL * N1; t += NT) {  // step B, straight to the rows
+    const int l = t / N1, k1 = t - l * N1;
+    const long q = q0 + l;
+    cpx<T> b[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) b[n2] = buf[l * LS + k1 * N2 + n2];
+    reg_fft<N2, +1>(b);
+    if (q < npairs) {
+      T *ra = grid + (2 * q) * RS;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        ra[k1 + N1 * k2] = b[k2].x;
+        ra[RS + k1 + N1 * k2] = b[k2].y;
+      }
+    }
+  }
+}
+
+// Complex FFT along y, in place.  Block = (x chunk of XW, plane z, comp).
+template <typename T, int K, int XW, int S, int NT>
+__device__ __forceinline__ void fft_y_block(T *__restrict__ grid, long comp_stride, int Khp,
+                                            int nxc, const cpx<T> *__restrict__ tw, int lb,
+                                            cpx<T> *buf, cpx<T> *s_tw) {
+  constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
+  constexpr int Kh = K / 2 + 1;
+  constexpr int LS = K + 1;
+  load_tw<T, K>(s_tw, tw);
+  __syncthreads();
+  const int xc = lb % nxc;
+  const int zc = lb / nxc;  // z + K*comp
+  const int comp = zc / K, z = zc - comp * K;
+  const int x0 = xc * XW;
+  cpx<T> *plane = reinterpret_cast<cpx<T> *>(grid + comp * comp_stride) + (size_t)z * K * Khp;
+  for (int t = threadIdx.x; t < XW * N2; t += NT) {  // step A from global
+    const int xo = t % XW, n2 = t / XW;
+    const int x = x0 + xo;
+    cpx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1)
+      a[n1] = x < Kh ? plane[(size_t)(n1 * N2 + n2) * Khp + x] : cpx<T>{T(0), T(0)};
+    reg_fft<N1, S>(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<S>(s_tw, n2 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) buf[xo * LS + k1 * N2 + n2] = a[k1];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < XW * N1; t += NT) {  // step B to global
+    const int xo = t % XW, k1 = t / XW;
+    const int x = x0 + xo;
+    cpx<T> b[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; ++n2) b[n2] = buf[xo * LS + k1 * N2 + n2];
+    reg_fft<N2, S>(b);
+    if (x < Kh) {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) plane[(size_t)(k1 + N1 * k2) * Khp + x] = b[k2];
+    }
+  }
+}
+
+// Forward FFT along z, RPY influence (old-version/spme_kernel.c:73-228),
+// backward FFT along z, for the 3 components of one right-hand side.
+// Block = (x chunk, y) x rhs.  tab[(z*K + y)*Khp + x] = psi(|k|)|b(k)|^2/V.
+// Wave numbers follow old-version/spme.c:95-107 (index K/2 -> +pi K/L).
+// The spectrum of a (column, k1) stays in registers between the forward
+// N2-point FFT, the 3x3 influence and the backward N2-point FFT.
+template <typename T, int K, int XW, int NT>
+__device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp_stride,
+                                                 int Khp, int nxc, const T *__restrict__ tab,
+                                                 T dk, const cpx<T> *__restrict__ tw, int lb,
+                                                 cpx<T> *buf, cpx<T> *s_tw, T *s_tab) {
+  constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
+  constexpr int Kh = K / 2 + 1;
+  constexpr int LS = K + 1;
+  load_tw<T, K>(s_tw, tw);
+  // adjacent x chunks (halves of one cache line) on the same XCD
+  const int xc = lb % nxc, y = (lb / nxc) % K;
+  const int x0 = xc * XW;
+  const int rhs = lb / (nxc * K);
+  for (int t = threadIdx.x; t < XW * K; t += NT) {
+    const int z = t / XW, xo = t - z * XW;
+    const int x = x0 + xo;
+    s_tab[xo * K + z] = x < Kh ? tab[((size_t)z * K + y) * Khp + x] : T(0);
+  }
+  __syncthreads();
+  cpx<T> *g0 = reinterpret_cast<cpx<T> *>(grid + (3L * rhs) * comp_stride);
+  const long cs = comp_stride / 2;  // complex elements
+  for (int t = threadIdx.x; t < 3 * XW * N2; t += NT) {  // step A (forward) from global
+    const int xo = t % XW, r = t / XW;
+    const int c = r % 3, n2 = r / 3;
+    const int x = x0 + xo;
+    cpx<T> a[N1];
+#pragma unroll
+    for (int n1 = 0; n1 < N1; ++n1)
+      a[n1] = x < Kh ? g0[c * cs + ((size_t)(n1 * N2 + n2) * K + y) * Khp + x]
+                     : cpx<T>{T(0), T(0)};
+    reg_fft<N1, -1>(a);
+#pragma unroll
+    for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<-1>(s_tw, n2 * k1));
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) buf[(c * XW + xo) * LS + k1 * N2 + n2] = a[k1];
+  }
+  __syncthreads();
+  const T ky = T(y <= K / 2 ? y : y - K) * dk;
+  for (int t = threadIdx.x; t < XW * N1; t += NT) {  // step B, influence, inverse step B
+    const int xo = t % XW, k1 = t / XW;
+    const int x = x0 + xo;
+    cpx<T> b[3][N2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) b[c][n2] = buf[(c * XW + xo) * LS + k1 * N2 + n2];
+      reg_fft<N2, -1>(b[c]);
+    }
+    const T kx = T(x) * dk;  // x <= K/2 in the half spectrum
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const int z = k1 + N1 * k2;
+      const T kz = T(z <= K / 2 ? z : z - K) * dk;
+      const T k2n = kx * kx + ky * ky + kz * kz;
+      const T m2 = s_tab[xo * K + z];
+      // k = 0 and padding columns carry m2 = 0; guard the division there
+      const T ik = k2n > T(0) ? T(1) / k2n : T(0);
+      const T B0 = m2 - m2 * kx * kx * ik, B1 = -m2 * kx * ky * ik, B2 = -m2 * kx * kz * ik;
+      const T B3 = m2 - m2 * ky * ky * ik, B4 = -m2 * ky * kz * ik, B5 = m2 - m2 * kz * kz * ik;
+      const cpx<T> va = b[0][k2], vb = b[1][k2], vc = b[2][k2];
+      b[0][k2] = {B0 * va.x + B1 * vb.x + B2 * vc.x, B0 * va.y + B1 * vb.y + B2 * vc.y};
+      b[1][k2] = {B1 * va.x + B3 * vb.x + B4 * vc.x, B1 * va.y + B3 * vb.y + B4 * vc.y};
+      b[2][k2] = {B2 * va.x + B4 * vb.x + B5 * vc.x, B2 * va.y + B4 * vb.y + B5 * vc.y};
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      reg_fft<N2, +1>(b[c]);
+#pragma unroll
+      for (int n2 = 1; n2 < N2; ++n2) b[c][n2] = cmul(b[c][n2], twiddle<+1>(s_tw, n2 * k1));
+#pragma unroll
+      for (int n2 = 0; n2 < N2; ++n2) buf[(c * XW + xo) * LS + k1 * N2 + n2] = b[c][n2];
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 3 * XW * N2; t += NT) {  // inverse step A to global
+    const int xo = t % XW, r = t / XW;
+    const int c = r % 3, n2 = r / 3;
+    const int x = x0 + xo;
+    cpx<T> a[N1];
+#pragma unroll
+    for (int k1 = 0; k1 < N1; ++k1) a[k1] = buf[(c * XW + xo) * LS + k1 * N2 + n2];
+    reg_fft<N1, +1>(a);
+    if (x < Kh) {
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1)
+        g0[c * cs + ((size_t)(n2 + N2 * n1) * K + y) * Khp + x] = a[n1];
+    }
+  }
+}
+
+// One row (cell order) of the real-space SpMV.  fs: forces in cell order [N][C].
+// out[r*ldout + 3*perm[s] + d] = (accum ? out : beta*out) + alpha*(self f + sum_j B_sj f_j).
+// Rows are packed in slices of 64 (entry-major within a slice).
+template <typename T, int C>
+__device__ __forceinline__ void real_row(const int *__restrict__ perm,
+                                         const int *__restrict__ width,
+                                         const int *__restrict__ soff,
+                                         const int *__restrict__ col,
+                                         const T *__restrict__ blk, long E, T self,
+                                         const T *__restrict__ fs, T alpha, T beta, int accum,
+                                         T *__restrict__ out, long ldout, int s) {
+  const int q = s >> 6, l = s & 63;
+  T acc[C];
+#pragma unroll
+  for (int c = 0; c < C; ++c) acc[c] = self * fs[(size_t)s * C + c];
+  const long base = soff[q] + l;
+  const int w = width[q];
+#pragma unroll 4
+  for (int k = 0; k < w; ++k) {
+    const long e = base + 64L * k;
+    const int j = col[e];
+    const T b0 = blk[e], b1 = blk[E + e], b2 = blk[2 * E + e];
+    const T b3 = blk[3 * E + e], b4 = blk[4 * E + e], b5 = blk[5 * E + e];
+    const T *fj = fs + (size_t)j * C;
+#pragma unroll
+    for (int r = 0; r < C / 3; ++r) {
+      const T f0 = fj[3 * r], f1 = fj[3 * r + 1], f2 = fj[3 * r + 2];
+      acc[3 * r + 0] += b0 * f0 + b1 * f1 + b2 * f2;
+      acc[3 * r + 1] += b1 * f0 + b3 * f1 + b4 * f2;
+      acc[3 * r + 2] += b2 * f0 + b4 * f1 + b5 * f2;
+    }
+  }
+  const int o = perm[s];
+#pragma unroll
+  for (int c = 0; c < C; ++c) {
+    const int r = c / 3, d = c - 3 * r;
+    T *u = out + r * ldout + 3L * o + d;
+    const T prev = accum ? *u : (beta == T(0) ? T(0) : beta * *u);
+    *u = prev + alpha * acc[c];
+  }
+}
+
+}  // namespace spme

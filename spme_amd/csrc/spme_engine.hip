@@ -12,12 +12,14 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/spme.h"
 #include "../../include/spme_ext.h"
+#include "spme_group.h"
 #include "spme_kernels.h"
 #include "spme_launch.h"
 #include "spme_real.h"
@@ -155,6 +157,10 @@ struct _spme_t {
   std::vector<hipEvent_t> pool;
   double stage_ms[SPME_NSTAGES];
   int stage_n[SPME_NSTAGES];
+  // requested tile shapes (0 = default) and a counter of position sets, so
+  // that groups holding this engine notice a rebuild of its buffers
+  spme_tiles_t tiles;
+  unsigned pos_version;
 };
 
 namespace {
@@ -356,6 +362,7 @@ int set_positions(_spme_t *e, const double *pos, int on_device, hipStream_t st) 
   HIP_TRY(hipStreamSynchronize(st));
   if (herr) return fail(-1, "positions must be finite and within [-L, 2L)");
   e->have_pos = true;
+  ++e->pos_version;
   return 0;
 }
 
@@ -388,10 +395,37 @@ int collect_timing(_spme_t *e) {
   return 0;
 }
 
-int pick_cc(int C, int P, size_t tsz) {
-  (void)P;
+int pick_cc(int C, size_t tsz) {
   if (tsz == 4 && C % 12 == 0) return 12;
   return C % 6 == 0 ? 6 : 3;
+}
+
+// Tile shapes of one system for C components, defaults resolved.
+struct Tiles {
+  int spread_cc, spread_zc, interp_zc;
+};
+
+Tiles resolve_tiles(const _spme_t *e, const spme_tiles_t &t, int C) {
+  const int K = e->K;
+  Tiles r;
+  r.spread_cc = (t.spread_cc > 0 && C % t.spread_cc == 0) ? t.spread_cc : pick_cc(C, e->tsz);
+  r.spread_zc = t.spread_zc > 0 ? std::min(t.spread_zc, K) : (K >= 64 ? K / 4 : K / 2);
+  const int izmax = std::min(K, InterpShape<float>::MAXZC);
+  r.interp_zc = t.interp_zc > 0 ? std::min(t.interp_zc, izmax) : std::min(K >= 64 ? K / 4 : K, izmax);
+  return r;
+}
+
+int check_tiles(const spme_tiles_t &t) {
+  if (!(t.spread_cc == 0 || t.spread_cc == 3 || t.spread_cc == 6 || t.spread_cc == 12))
+    return fail(-1, "spread_cc must be 0, 3, 6 or 12 (got %d)", t.spread_cc);
+  if (t.spread_zc < 0 || t.spread_zc > SPREAD_MAXZC)
+    return fail(-1, "spread_zc must be in [0, %d] (got %d)", SPREAD_MAXZC, t.spread_zc);
+  if (t.interp_zc < 0 || t.interp_zc > InterpShape<float>::MAXZC)
+    return fail(-1, "interp_zc must be in [0, %d] (got %d)", InterpShape<float>::MAXZC,
+                t.interp_zc);
+  for (int v : {t.spread_persist, t.interp_persist, t.real_persist})
+    if (v < 0 || v > 64) return fail(-1, "persist values must be in [0, 64] (got %d)", v);
+  return 0;
 }
 
 template <typename T>
@@ -415,12 +449,12 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
     rec.marks.push_back({stage, ev});
     return 0;
   };
+  const Tiles tl = resolve_tiles(e, e->tiles, C);
   if (parts & SPME_RECIP) {
     launch_permute_in<T>(e->perm, in, ldin, (T *)e->fs, N, C, st);
     mark(SPME_ST_PERMUTE);
-    SpreadArgs<T> sa{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, C, pick_cc(C, e->P, e->tsz),
-                     e->P, (T *)e->grid, e->comp_stride, e->RS, e->K,
-                     e->K >= 64 ? e->K / 4 : e->K / 2};
+    SpreadArgs<T> sa{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, C, tl.spread_cc,
+                     e->P, (T *)e->grid, e->comp_stride, e->RS, e->K, tl.spread_zc};
     if (launch_spread<T>(sa, st)) return fail(-1, "spread: unsupported configuration");
     mark(SPME_ST_SPREAD);
     if (stop_after == 1) return 0;
@@ -434,8 +468,9 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
     if (timed)
       for (int i = 0; i < 5; ++i) rec.marks.push_back({SPME_ST_FFT_X_FWD + i, fev[i]});
     if (stop_after == 2) return 0;
-    InterpArgs<T> ia{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid, e->comp_stride,
-                     e->RS, e->K, N, C, e->P, T(alpha), T(beta), out, ldout};
+    InterpArgs<T> ia{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid,
+                     e->comp_stride, e->RS, e->K, N, C, e->P, tl.interp_zc, T(alpha), T(beta),
+                     out, ldout};
     if (launch_interp<T>(ia, st)) return fail(-1, "interpolate: unsupported configuration");
     mark(SPME_ST_INTERP);
   }
@@ -474,6 +509,493 @@ int apply(_spme_t *e, int nrhs, double alpha, const void *in, long ldin, double 
                           parts, st, stop_after);
   return apply_t<double>(e, nrhs, alpha, (const double *)in, ldin, beta, (double *)out, ldout,
                          parts, st, stop_after);
+}
+
+// ---------------------------------------------------------------------------
+// Autotuning: candidate shapes are timed with HIP events on the caller's
+// stream.  Every candidate computes the same bits, so the choice is purely
+// by time.
+
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  int init() {
+    HIP_TRY(hipEventCreate(&a));
+    HIP_TRY(hipEventCreate(&b));
+    return 0;
+  }
+};
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+// Mean milliseconds per call of launch() (which returns 0 or an error code)
+// over reps timed calls after one untimed call.
+template <class F>
+int time_launch(const EventPair &ev, hipStream_t st, int reps, F &&launch, float &ms) {
+  if (launch()) return fail(-1, "autotune: launch rejected");
+  HIP_TRY(hipEventRecord(ev.a, st));
+  for (int r = 0; r < reps; ++r)
+    if (launch()) return fail(-1, "autotune: launch rejected");
+  HIP_TRY(hipEventRecord(ev.b, st));
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventSynchronize(ev.b));
+  HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+  ms /= float(reps);
+  return 0;
+}
+
+// z-chunk candidates: K/d for small d, within [lo, hi].
+std::vector<int> zc_candidates(int K, int lo, int hi) {
+  std::vector<int> v;
+  for (int d : {1, 2, 3, 4, 6, 8}) {
+    const int z = (K + d - 1) / d;
+    if (z >= lo && z <= hi && std::find(v.begin(), v.end(), z) == v.end()) v.push_back(z);
+  }
+  return v;
+}
+
+template <typename T>
+int autotune_t(_spme_t *e, int nrhs, int reps, hipStream_t st) {
+  const int C = 3 * nrhs, K = e->K, N = e->N;
+  EventPair ev;
+  if (int rc = ev.init()) return rc;
+  DevBuf out;
+  HIP_TRY(hipMalloc(&out.p, sizeof(T) * (size_t)C * N));
+  HIP_TRY(hipMemsetAsync(e->fs, 0, e->tsz * (size_t)C * N, st));
+  spme_tiles_t best = e->tiles;
+  float best_ms = 1e30f;
+  for (int cc : {3, 6, 12}) {
+    if (C % cc) continue;
+    for (int zc : zc_candidates(K, 8, SPREAD_MAXZC)) {
+      SpreadArgs<T> sa{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, C, cc, e->P,
+                       (T *)e->grid, e->comp_stride, e->RS, K, zc};
+      float ms = 0.f;
+      if (int rc = time_launch(ev, st, reps, [&] { return launch_spread<T>(sa, st); }, ms))
+        return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best.spread_cc = cc;
+        best.spread_zc = zc;
+      }
+    }
+  }
+  best_ms = 1e30f;
+  for (int zc : zc_candidates(K, 4, std::min(K, (int)InterpShape<T>::MAXZC))) {
+    InterpArgs<T> ia{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid,
+                     e->comp_stride, e->RS, K, N, C, e->P, zc, T(1), T(0), (T *)out.p,
+                     3L * N};
+    float ms = 0.f;
+    if (int rc = time_launch(ev, st, reps, [&] { return launch_interp<T>(ia, st); }, ms))
+      return rc;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best.interp_zc = zc;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  e->tiles = best;
+  return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Groups of independent systems.  Stage order of a grouped apply: force
+// permutation (both orderings), spread, the five FFT passes, interpolation,
+// real space; one launch each.  Per stage, the logical blocks of all
+// systems are numbered system after system; maps[stage] holds pre (nsys+1
+// first blocks) followed by the system of every block.
+
+enum { GS_PERM = 0, GS_SPREAD, GS_FFT0, GS_INTERP = GS_FFT0 + 5, GS_REAL, GS_N };
+
+struct _spme_group_t {
+  std::vector<_spme_t *> sys;
+  std::vector<unsigned> ver;  // position version of each engine at the last map build
+  int nrhs = 0, prec = 0, P = 0, kmax = 96, txy = 16, ncu = 256;
+  size_t tsz = 4;
+  spme_tiles_t tiles{};
+  // block maps (device)
+  int *d_maps = nullptr;
+  size_t maps_cap = 0;
+  int total[GS_N] = {};
+  const int *pre[GS_N] = {};
+  const int *smap[GS_N] = {};
+  // argument tables: device copy and pinned host staging
+  char *d_args = nullptr, *h_args = nullptr;
+  size_t args_cap = 0;
+  size_t a_off[5] = {};  // perm, spread, fft, interp, real
+  hipEvent_t up_ev = nullptr;
+  hipStream_t last = nullptr;
+  bool used = false, maps_ok = false, args_ok = false;
+  // the call the uploaded arguments describe
+  std::vector<const void *> k_in;
+  std::vector<void *> k_out;
+  std::vector<long> k_ldin, k_ldout;
+  double k_alpha = 0, k_beta = 0;
+  int k_parts = 0;
+  // captured apply
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_parts = -1;
+};
+
+namespace {
+
+void group_drop_graph(_spme_group_t *g) {
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->exec = nullptr;
+  g->graph = nullptr;
+  g->graph_parts = -1;
+}
+
+// Waits for every earlier use of the group's tables.
+int group_quiesce(_spme_group_t *g) {
+  if (g->used) HIP_TRY(hipStreamSynchronize(g->last));
+  if (g->up_ev) HIP_TRY(hipEventSynchronize(g->up_ev));
+  return 0;
+}
+
+int group_stage_blocks(const _spme_group_t *g, const _spme_t *e, int stage) {
+  const int C = 3 * g->nrhs;
+  const Tiles tl = resolve_tiles(e, g->tiles, C);
+  switch (stage) {
+    case GS_PERM: return group_perm_blocks(e->N, C);
+    case GS_SPREAD: return spread_blocks(e->K, tl.spread_zc, C, tl.spread_cc);
+    case GS_INTERP:
+      return interp_blocks(e->K, tl.interp_zc, C,
+                           g->tsz == 4 ? InterpShape<float>::CC : InterpShape<double>::CC, g->txy);
+    case GS_REAL: return group_real_blocks(e->N);
+    default:
+      return g->tsz == 4 ? fft_pass_blocks<float>(e->K, C, stage - GS_FFT0)
+                         : fft_pass_blocks<double>(e->K, C, stage - GS_FFT0);
+  }
+}
+
+int group_build_maps(_spme_group_t *g) {
+  const int n = (int)g->sys.size();
+  std::vector<int> h;
+  size_t off[GS_N];
+  for (int stage = 0; stage < GS_N; ++stage) {
+    off[stage] = h.size();
+    long acc = 0;
+    std::vector<int> cnt(n);
+    for (int s = 0; s < n; ++s) {
+      cnt[s] = group_stage_blocks(g, g->sys[s], stage);
+      h.push_back(int(acc));
+      acc += cnt[s];
+      if (acc > (1L << 30)) return fail(-1, "group too large: stage %d has %ld blocks", stage, acc);
+    }
+    h.push_back(int(acc));
+    for (int s = 0; s < n; ++s) h.insert(h.end(), cnt[s], s);
+    g->total[stage] = int(acc);
+  }
+  if (int rc = group_quiesce(g)) return rc;
+  group_drop_graph(g);
+  if (h.size() > g->maps_cap) {
+    if (g->d_maps) (void)hipFree(g->d_maps);
+    g->d_maps = nullptr;
+    g->maps_cap = 0;
+    HIP_TRY(hipMalloc(&g->d_maps, sizeof(int) * h.size()));
+    g->maps_cap = h.size();
+  }
+  HIP_TRY(hipMemcpy(g->d_maps, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
+  for (int stage = 0; stage < GS_N; ++stage) {
+    g->pre[stage] = g->d_maps + off[stage];
+    g->smap[stage] = g->d_maps + off[stage] + n + 1;
+  }
+  for (int s = 0; s < n; ++s) g->ver[s] = g->sys[s]->pos_version;
+  g->maps_ok = true;
+  g->args_ok = false;
+  return 0;
+}
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+template <typename T>
+int group_build_args(_spme_group_t *g, double alpha, const void *const *in, const long *ldin,
+                     double beta, void *const *out, const long *ldout, int parts,
+                     hipStream_t st) {
+  const int n = (int)g->sys.size(), C = 3 * g->nrhs;
+  const size_t sz[5] = {sizeof(GPermArgs<T>), sizeof(SpreadArgs<T>), sizeof(FftArgs<T>),
+                        sizeof(InterpArgs<T>), sizeof(RealArgs<T>)};
+  size_t bytes = 0;
+  for (int k = 0; k < 5; ++k) {
+    g->a_off[k] = bytes;
+    bytes += align256(sz[k] * n);
+  }
+  if (int rc = group_quiesce(g)) return rc;
+  if (bytes > g->args_cap) {
+    group_drop_graph(g);
+    if (g->d_args) (void)hipFree(g->d_args);
+    if (g->h_args) (void)hipHostFree(g->h_args);
+    g->d_args = g->h_args = nullptr;
+    g->args_cap = 0;
+    HIP_TRY(hipMalloc(&g->d_args, bytes));
+    HIP_TRY(hipHostMalloc(&g->h_args, bytes));
+    g->args_cap = bytes;
+  }
+  auto *pa = reinterpret_cast<GPermArgs<T> *>(g->h_args + g->a_off[0]);
+  auto *sa = reinterpret_cast<SpreadArgs<T> *>(g->h_args + g->a_off[1]);
+  auto *fa = reinterpret_cast<FftArgs<T> *>(g->h_args + g->a_off[2]);
+  auto *ia = reinterpret_cast<InterpArgs<T> *>(g->h_args + g->a_off[3]);
+  auto *ra = reinterpret_cast<RealArgs<T> *>(g->h_args + g->a_off[4]);
+  for (int s = 0; s < n; ++s) {
+    _spme_t *e = g->sys[s];
+    const Tiles tl = resolve_tiles(e, g->tiles, C);
+    const bool rec = parts & SPME_RECIP, real = parts & SPME_REAL;
+    pa[s] = GPermArgs<T>{rec ? e->perm : nullptr, real ? e->r_perm : nullptr,
+                         (const T *)in[s], ldin[s], (T *)e->fs, (T *)e->r_fs, e->N, C};
+    sa[s] = SpreadArgs<T>{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, C,
+                          tl.spread_cc, e->P, (T *)e->grid, e->comp_stride, e->RS, e->K,
+                          tl.spread_zc};
+    fa[s] = FftArgs<T>{(T *)e->grid, e->comp_stride, e->RS, e->K, C, (const T *)e->tab,
+                       T(2.0 * M_PI / e->prm.box_size), (const cpx<T> *)e->tw};
+    ia[s] = InterpArgs<T>{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid,
+                          e->comp_stride, e->RS, e->K, e->N, C, e->P, tl.interp_zc, T(alpha),
+                          T(beta), (T *)out[s], ldout[s]};
+    const double xi = e->prm.xi;
+    const double self = (parts & SPME_SELF) ? 1.0 - 6.0 * xi / std::sqrt(M_PI) +
+                                                  40.0 * xi * xi * xi / (3.0 * std::sqrt(M_PI))
+                                            : 0.0;
+    ra[s] = RealArgs<T>{e->r_perm, e->r_width, e->r_soff, e->r_col, (const T *)e->r_blk,
+                        e->r_E, self, (const T *)e->r_fs, e->N, C, alpha, beta, rec ? 1 : 0,
+                        (T *)out[s], ldout[s]};
+  }
+  HIP_TRY(hipMemcpyAsync(g->d_args, g->h_args, bytes, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipEventRecord(g->up_ev, st));
+  g->last = st;
+  g->used = true;
+  g->k_in.assign(in, in + n);
+  g->k_out.assign(out, out + n);
+  g->k_ldin.assign(ldin, ldin + n);
+  g->k_ldout.assign(ldout, ldout + n);
+  g->k_alpha = alpha;
+  g->k_beta = beta;
+  g->k_parts = parts;
+  g->args_ok = true;
+  return 0;
+}
+
+// Hardware grid of a stage for a persistence setting (blocks per CU).
+int group_grid(const _spme_group_t *g, int stage, int persist) {
+  const int total = g->total[stage];
+  if (persist <= 0) return total;
+  const long want = ((long)persist * g->ncu + 7) / 8 * 8;
+  return want < total ? int(want) : total;
+}
+
+template <typename T>
+int group_launch(_spme_group_t *g, int stage, hipStream_t st) {
+  const spme_tiles_t &tl = g->tiles;
+  const char *A = g->d_args;
+  if (g->total[stage] == 0) return 0;
+  switch (stage) {
+    case GS_PERM: {
+      GroupStage<GPermArgs<T>> t{(const GPermArgs<T> *)(A + g->a_off[0]), g->pre[stage],
+                                 g->smap[stage], g->total[stage]};
+      launch_group_permute<T>(t, group_grid(g, stage, tl.real_persist), st);
+      return 0;
+    }
+    case GS_SPREAD: {
+      GroupStage<SpreadArgs<T>> t{(const SpreadArgs<T> *)(A + g->a_off[1]), g->pre[stage],
+                                  g->smap[stage], g->total[stage]};
+      const int cc = resolve_tiles(g->sys[0], tl, 3 * g->nrhs).spread_cc;
+      return launch_group_spread<T>(t, g->P, cc, group_grid(g, stage, tl.spread_persist), st);
+    }
+    case GS_INTERP: {
+      GroupStage<InterpArgs<T>> t{(const InterpArgs<T> *)(A + g->a_off[3]), g->pre[stage],
+                                  g->smap[stage], g->total[stage]};
+      return launch_group_interp<T>(t, g->P, g->txy, group_grid(g, stage, tl.interp_persist),
+                                    st);
+    }
+    case GS_REAL: {
+      GroupStage<RealArgs<T>> t{(const RealArgs<T> *)(A + g->a_off[4]), g->pre[stage],
+                                g->smap[stage], g->total[stage]};
+      return launch_group_real<T>(t, 3 * g->nrhs, group_grid(g, stage, tl.real_persist), st);
+    }
+    default: {
+      GroupStage<FftArgs<T>> t{(const FftArgs<T> *)(A + g->a_off[2]), g->pre[stage],
+                               g->smap[stage], g->total[stage]};
+      return launch_group_fft<T>(t, stage - GS_FFT0, g->kmax, st);
+    }
+  }
+}
+
+template <typename T>
+int group_launch_all(_spme_group_t *g, int parts, hipStream_t st) {
+  if (int rc = group_launch<T>(g, GS_PERM, st)) return rc;
+  if (parts & SPME_RECIP)
+    for (int stage = GS_SPREAD; stage <= GS_INTERP; ++stage)
+      if (int rc = group_launch<T>(g, stage, st)) return rc;
+  if (parts & SPME_REAL)
+    if (int rc = group_launch<T>(g, GS_REAL, st)) return rc;
+  return 0;
+}
+
+int group_check(_spme_group_t *g, int parts) {
+  if (parts == 0 || (parts & ~SPME_FULL)) return fail(-1, "bad parts mask %d", parts);
+  if ((parts & SPME_SELF) && !(parts & SPME_REAL))
+    return fail(-1, "SPME_SELF without SPME_REAL is not supported");
+  for (size_t s = 0; s < g->sys.size(); ++s) {
+    _spme_t *e = g->sys[s];
+    if (!e->have_pos) return fail(-1, "system %zu: positions have not been set", s);
+    if ((parts & SPME_REAL) && e->nc < 3)
+      return fail(-1, "system %zu: real-space part needs rc > 0 and box_size >= 3*max(rc, 2)",
+                  s);
+    if (e->pos_version != g->ver[s]) g->maps_ok = false;
+  }
+  return 0;
+}
+
+template <typename T>
+int group_apply_t(_spme_group_t *g, double alpha, const void *const *in, const long *ldin,
+                  double beta, void *const *out, const long *ldout, int parts, int graph,
+                  hipStream_t st) {
+  const int n = (int)g->sys.size();
+  for (int s = 0; s < n; ++s) {
+    if (!in[s] || !out[s]) return fail(-1, "system %d: null vector", s);
+    if (ldin[s] < 3L * g->sys[s]->N || ldout[s] < 3L * g->sys[s]->N)
+      return fail(-1, "system %d: ldin/ldout must be >= 3*np", s);
+  }
+  if (int rc = group_check(g, parts)) return rc;
+  if (!g->maps_ok)
+    if (int rc = group_build_maps(g)) return rc;
+  const bool same = g->args_ok && g->k_alpha == alpha && g->k_beta == beta &&
+                    g->k_parts == parts && std::equal(in, in + n, g->k_in.begin()) &&
+                    std::equal(out, out + n, g->k_out.begin()) &&
+                    std::equal(ldin, ldin + n, g->k_ldin.begin()) &&
+                    std::equal(ldout, ldout + n, g->k_ldout.begin());
+  if (g->used && g->last != st) HIP_TRY(hipStreamSynchronize(g->last));
+  if (!same)
+    if (int rc = group_build_args<T>(g, alpha, in, ldin, beta, out, ldout, parts, st))
+      return rc;
+  g->last = st;
+  g->used = true;
+  if (!graph) {
+    if (int rc = group_launch_all<T>(g, parts, st)) return rc;
+    HIP_TRY(hipGetLastError());
+    return 0;
+  }
+  if (!g->exec || g->graph_parts != parts) {
+    group_drop_graph(g);
+    HIP_TRY(hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal));
+    const int rc = group_launch_all<T>(g, parts, g->cap);
+    hipGraph_t gr = nullptr;
+    const hipError_t ec = hipStreamEndCapture(g->cap, &gr);
+    if (rc || ec != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc ? rc : fail(-2, "graph capture failed: %s", hipGetErrorString(ec));
+    }
+    g->graph = gr;
+    HIP_TRY(hipGraphInstantiate(&g->exec, gr, nullptr, nullptr, 0));
+    g->graph_parts = parts;
+  }
+  HIP_TRY(hipGraphLaunch(g->exec, st));
+  return 0;
+}
+
+template <typename T>
+int group_autotune_t(_spme_group_t *g, int parts, int reps, hipStream_t st) {
+  const int n = (int)g->sys.size(), C = 3 * g->nrhs;
+  if (int rc = group_check(g, parts)) return rc;
+  // scratch inputs (zero) and outputs, one slice per system
+  std::vector<long> ld(n), off(n);
+  long tot = 0;
+  for (int s = 0; s < n; ++s) {
+    ld[s] = 3L * g->sys[s]->N;
+    off[s] = tot;
+    tot += ld[s] * g->nrhs;
+  }
+  DevBuf din, dout;
+  HIP_TRY(hipMalloc(&din.p, sizeof(T) * tot));
+  HIP_TRY(hipMalloc(&dout.p, sizeof(T) * tot));
+  HIP_TRY(hipMemsetAsync(din.p, 0, sizeof(T) * tot, st));
+  std::vector<const void *> vin(n);
+  std::vector<void *> vout(n);
+  for (int s = 0; s < n; ++s) {
+    vin[s] = (const T *)din.p + off[s];
+    vout[s] = (T *)dout.p + off[s];
+  }
+  EventPair ev;
+  if (int rc = ev.init()) return rc;
+  g->last = st;
+  g->used = true;
+  auto run = [&](const spme_tiles_t &cand, int stage, float &ms) -> int {
+    g->tiles = cand;
+    if (int rc = group_build_maps(g)) return rc;
+    if (int rc = group_build_args<T>(g, 1.0, vin.data(), ld.data(), 0.0, vout.data(),
+                                     ld.data(), parts, st))
+      return rc;
+    if (int rc = group_launch<T>(g, GS_PERM, st)) return rc;
+    return time_launch(ev, st, reps, [&] { return group_launch<T>(g, stage, st); }, ms);
+  };
+  spme_tiles_t best = g->tiles;
+  float best_ms = 1e30f, ms = 0.f;
+  // spreading: components per pass and z chunk (0 = each system's default),
+  // then persistence
+  for (int cc : {3, 6, 12}) {
+    if (C % cc) continue;
+    for (int zc : {0, 16, 24, 32, 48, 64, 96}) {
+      spme_tiles_t c = best;
+      c.spread_cc = cc;
+      c.spread_zc = zc;
+      c.spread_persist = 0;
+      if (int rc = run(c, GS_SPREAD, ms)) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+  }
+  for (int ps : {1, 2, 4, 8}) {
+    spme_tiles_t c = best;
+    c.spread_persist = ps;
+    if (int rc = run(c, GS_SPREAD, ms)) return rc;
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  best_ms = 1e30f;
+  for (int zc : {0, 8, 16, 24, 32, 48, 64}) {
+    for (int ps : {0, 1, 2, 4}) {
+      spme_tiles_t c = best;
+      c.interp_zc = zc;
+      c.interp_persist = ps;
+      if (int rc = run(c, GS_INTERP, ms)) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+  }
+  if (parts & SPME_REAL) {
+    best_ms = 1e30f;
+    for (int ps : {0, 1, 2, 4, 8}) {
+      spme_tiles_t c = best;
+      c.real_persist = ps;
+      if (int rc = run(c, GS_REAL, ms)) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = c;
+      }
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  g->tiles = best;
+  g->maps_ok = false;  // rebuilt (and arguments re-uploaded) by the next apply
+  g->args_ok = false;
+  return 0;
 }
 
 void die(const char *where) {
@@ -595,6 +1117,139 @@ void cutoff2xi(double rmax, double ewald_eps, double box_size, double *_xi) {
     else hi = mid;
   }
   *_xi = hi;
+}
+
+int spme_engine_set_tiles(spme_t *spme, const spme_tiles_t *tiles) {
+  if (!spme) return fail(-1, "null engine");
+  const spme_tiles_t t = tiles ? *tiles : spme_tiles_t{};
+  if (int rc = check_tiles(t)) return rc;
+  spme->tiles = t;
+  return 0;
+}
+
+int spme_engine_get_tiles(const spme_t *spme, int nrhs, spme_tiles_t *tiles) {
+  if (!spme || !tiles) return fail(-1, "null argument");
+  if (nrhs < 1 || nrhs > spme->max_rhs)
+    return fail(-1, "nrhs must be in [1, %d] (got %d)", spme->max_rhs, nrhs);
+  const Tiles r = resolve_tiles(spme, spme->tiles, 3 * nrhs);
+  *tiles = spme->tiles;
+  tiles->spread_cc = r.spread_cc;
+  tiles->spread_zc = r.spread_zc;
+  tiles->interp_zc = r.interp_zc;
+  return 0;
+}
+
+int spme_engine_autotune(spme_t *spme, int nrhs, int reps, spme_tiles_t *best, void *stream) {
+  if (!spme) return fail(-1, "null engine");
+  (void)hipGetLastError();
+  if (!spme->have_pos) return fail(-1, "positions have not been set");
+  if (nrhs < 1 || nrhs > spme->max_rhs)
+    return fail(-1, "nrhs must be in [1, %d] (got %d)", spme->max_rhs, nrhs);
+  if (reps < 1) return fail(-1, "reps must be positive");
+  hipStream_t st = (hipStream_t)stream;
+  const int rc = spme->prec == SPME_FP32 ? autotune_t<float>(spme, nrhs, reps, st)
+                                         : autotune_t<double>(spme, nrhs, reps, st);
+  if (rc) return rc;
+  return best ? spme_engine_get_tiles(spme, nrhs, best) : 0;
+}
+
+int spme_group_create(int nsys, spme_t *const *engines, int nrhs, spme_group_t **out) {
+  if (!out || !engines) return fail(-1, "null argument");
+  *out = nullptr;
+  if (nsys < 1) return fail(-1, "nsys must be positive");
+  for (int s = 0; s < nsys; ++s) {
+    const spme_t *e = engines[s];
+    if (!e) return fail(-1, "engine %d is null", s);
+    if (e->prec != engines[0]->prec || e->P != engines[0]->P)
+      return fail(-1, "engine %d: all engines need one precision and one porder", s);
+    if (nrhs < 1 || nrhs > e->max_rhs)
+      return fail(-1, "engine %d: nrhs must be in [1, max_rhs = %d]", s, e->max_rhs);
+    for (int r = 0; r < s; ++r)
+      if (engines[r] == e) return fail(-1, "engine %d appears twice", s);
+  }
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  for (int s = 0; s < nsys; ++s)
+    if (engines[s]->device != dev) return fail(-1, "engine %d lives on another device", s);
+  _spme_group_t *g = new _spme_group_t();
+  g->sys.assign(engines, engines + nsys);
+  g->ver.assign(nsys, ~0u);
+  g->nrhs = nrhs;
+  g->prec = engines[0]->prec;
+  g->P = engines[0]->P;
+  g->tsz = engines[0]->tsz;
+  int kmax = 0;
+  bool all16 = true;
+  for (auto *e : g->sys) {
+    kmax = std::max(kmax, e->K);
+    all16 = all16 && e->K % 16 == 0;
+  }
+  g->kmax = group_kmax(kmax);
+  g->txy = all16 ? 16 : 8;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipEventCreateWithFlags(&g->up_ev, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking) != hipSuccess) {
+    spme_group_destroy(g);
+    return fail(-2, "group set-up failed");
+  }
+  g->ncu = ncu > 0 ? ncu : 256;
+  *out = g;
+  return 0;
+}
+
+int spme_group_destroy(spme_group_t *g) {
+  if (!g) return 0;
+  (void)group_quiesce(g);
+  group_drop_graph(g);
+  if (g->d_maps) (void)hipFree(g->d_maps);
+  if (g->d_args) (void)hipFree(g->d_args);
+  if (g->h_args) (void)hipHostFree(g->h_args);
+  if (g->up_ev) (void)hipEventDestroy(g->up_ev);
+  if (g->cap) (void)hipStreamDestroy(g->cap);
+  delete g;
+  return 0;
+}
+
+int spme_group_set_tiles(spme_group_t *g, const spme_tiles_t *tiles) {
+  if (!g) return fail(-1, "null group");
+  const spme_tiles_t t = tiles ? *tiles : spme_tiles_t{};
+  if (int rc = check_tiles(t)) return rc;
+  g->tiles = t;
+  g->maps_ok = false;
+  g->args_ok = false;
+  return 0;
+}
+
+int spme_group_get_tiles(const spme_group_t *g, spme_tiles_t *tiles) {
+  if (!g || !tiles) return fail(-1, "null argument");
+  *tiles = g->tiles;
+  return 0;
+}
+
+int spme_group_autotune(spme_group_t *g, int parts, int reps, spme_tiles_t *best,
+                        void *stream) {
+  if (!g) return fail(-1, "null group");
+  (void)hipGetLastError();
+  if (reps < 1) return fail(-1, "reps must be positive");
+  hipStream_t st = (hipStream_t)stream;
+  const int rc = g->prec == SPME_FP32 ? group_autotune_t<float>(g, parts, reps, st)
+                                      : group_autotune_t<double>(g, parts, reps, st);
+  if (rc) return rc;
+  if (best) *best = g->tiles;
+  return 0;
+}
+
+int spme_group_apply(spme_group_t *g, double alpha, const void *const *in, const long *ldin,
+                     double beta, void *const *out, const long *ldout, int parts, int graph,
+                     void *stream) {
+  if (!g) return fail(-1, "null group");
+  if (!in || !out || !ldin || !ldout) return fail(-1, "null argument");
+  (void)hipGetLastError();
+  hipStream_t st = (hipStream_t)stream;
+  return g->prec == SPME_FP32
+             ? group_apply_t<float>(g, alpha, in, ldin, beta, out, ldout, parts, graph, st)
+             : group_apply_t<double>(g, alpha, in, ldin, beta, out, ldout, parts, graph, st);
 }
 
 void create_spme_engine(double xi, int dim, int porder, int np, double box_size, int nthreads,

@@ -28,10 +28,32 @@ template <typename T> struct InterpArgs {
   const int *perm;
   const T *grid;
   long comp_stride;
-  int RS, K, N, C, P;
+  int RS, K, N, C, P, ZC;
   T alpha, beta;
   T *out;
   long ldout;
+};
+
+// Spread: one wave per 8 x 8 column tile, z chunk of ZC planes, CC
+// components per pass; SPREAD_MAXZC bounds ZC (LDS bucket table).
+constexpr int SPREAD_MAXZC = 256;
+inline int spread_blocks(int K, int ZC, int C, int CC) {
+  return (K / 8) * (K / 8) * ((K + ZC - 1) / ZC) * (C / CC);
+}
+// Interpolation: CC components per block over a TXY x TXY tile, z chunk of
+// ZC <= MAXZC planes (LDS bucket table).
+template <typename T> struct InterpShape {
+  static constexpr int CC = sizeof(T) == 4 ? 3 : 1;
+  static constexpr int MAXZC = 64;
+};
+inline int interp_blocks(int K, int ZC, int C, int CC, int TXY) {
+  return (K / TXY) * (K / TXY) * ((K + ZC - 1) / ZC) * (C / CC);
+}
+// FFT line batches: keep ~24 KB of LDS per block.
+template <typename T> struct FftShape {
+  static constexpr int NLX = sizeof(T) == 8 ? 8 : 16;  // complex lines per x block
+  static constexpr int XW = 16;                        // y-pass columns
+  static constexpr int XWZ = sizeof(T) == 8 ? 4 : 8;   // z-pass columns (x3 comps)
 };
 
 template <typename T> struct FftArgs {

@@ -15,6 +15,7 @@
 // pairs the 1/r^3 terms of M_real and of the far-field subtraction cancel.
 #include <hip/hip_runtime.h>
 
+#include "spme_bodies.h"
 #include "spme_kernels.h"
 #include "spme_real.h"
 
@@ -188,8 +189,7 @@ __global__ __launch_bounds__(256) void k_pair_fill(const double *__restrict__ sp
   }
 }
 
-// One thread per row (cell order).  fs: forces in cell order [N][C].
-// out[r*ldout + 3*perm[s] + d] = (accum ? out : beta*out) + alpha*(self f + sum_j B_sj f_j).
+// One thread per row (cell order).
 template <typename T, int C>
 __global__ __launch_bounds__(256) void k_real_spmv(const int *__restrict__ perm,
                                                    const int *__restrict__ width,
@@ -200,36 +200,8 @@ __global__ __launch_bounds__(256) void k_real_spmv(const int *__restrict__ perm,
                                                    T alpha, T beta, int accum,
                                                    T *__restrict__ out, long ldout) {
   const int s = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (s >= N) return;
-  const int q = s >> 6, l = s & 63;
-  T acc[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) acc[c] = self * fs[(size_t)s * C + c];
-  const long base = soff[q] + l;
-  const int w = width[q];
-#pragma unroll 4
-  for (int k = 0; k < w; ++k) {
-    const long e = base + 64L * k;
-    const int j = col[e];
-    const T b0 = blk[e], b1 = blk[E + e], b2 = blk[2 * E + e];
-    const T b3 = blk[3 * E + e], b4 = blk[4 * E + e], b5 = blk[5 * E + e];
-    const T *fj = fs + (size_t)j * C;
-#pragma unroll
-    for (int r = 0; r < C / 3; ++r) {
-      const T f0 = fj[3 * r], f1 = fj[3 * r + 1], f2 = fj[3 * r + 2];
-      acc[3 * r + 0] += b0 * f0 + b1 * f1 + b2 * f2;
-      acc[3 * r + 1] += b1 * f0 + b3 * f1 + b4 * f2;
-      acc[3 * r + 2] += b2 * f0 + b4 * f1 + b5 * f2;
-    }
-  }
-  const int o = perm[s];
-#pragma unroll
-  for (int c = 0; c < C; ++c) {
-    const int r = c / 3, d = c - 3 * r;
-    T *u = out + r * ldout + 3L * o + d;
-    const T prev = accum ? *u : (beta == T(0) ? T(0) : beta * *u);
-    *u = prev + alpha * acc[c];
-  }
+  if (s < N)
+    real_row<T, C>(perm, width, soff, col, blk, E, self, fs, alpha, beta, accum, out, ldout, s);
 }
 
 static int grid1(long n, int b) { return int((n + b - 1) / b); }

@@ -17,9 +17,10 @@ import numpy as np
 
 from . import lib as _L
 from .lib import (SPME_FP32, SPME_FP64, SPME_FULL, SPME_REAL, SPME_RECIP,
-                  SPME_SELF, SpmeError, SpmeInfo, SpmeParams, check, load)
+                  SPME_SELF, SpmeError, SpmeInfo, SpmeParams, SpmeTiles, check,
+                  load)
 
-__all__ = ["Engine", "DropIn", "cutoff2xi", "SPME_FP32", "SPME_FP64",
+__all__ = ["Engine", "Group", "DropIn", "cutoff2xi", "SPME_FP32", "SPME_FP64",
            "SPME_FULL", "SPME_REAL", "SPME_RECIP", "SPME_SELF", "SpmeError"]
 
 
@@ -119,6 +120,28 @@ class Engine:
             int(u.stride(0)), int(parts), ctypes.c_void_p(s)), "apply")
         return u
 
+    def set_tiles(self, tiles=None):
+        """tiles: dict of spme_tiles_t fields (missing = 0 = default), or None."""
+        t = _tiles(tiles)
+        check(self._lib.spme_engine_set_tiles(
+            self._h, ctypes.byref(t) if t is not None else None), "set_tiles")
+
+    def tiles(self, nrhs=1):
+        t = SpmeTiles()
+        check(self._lib.spme_engine_get_tiles(self._h, int(nrhs), ctypes.byref(t)),
+              "get_tiles")
+        return t.as_dict()
+
+    def autotune(self, nrhs=1, reps=5, stream=None):
+        """Time the candidate tile shapes, keep and return the fastest."""
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        t = SpmeTiles()
+        check(self._lib.spme_engine_autotune(self._h, int(nrhs), int(reps),
+                                             ctypes.byref(t), ctypes.c_void_p(s)),
+              "autotune")
+        return t.as_dict()
+
     def set_timing(self, enable=True):
         check(self._lib.spme_engine_set_timing(self._h, int(bool(enable))),
               "set_timing")
@@ -143,6 +166,82 @@ class Engine:
             self._h, int(stage), int(nrhs), ctypes.c_void_p(f.data_ptr()),
             int(f.stride(0)), out.ctypes.data_as(ctypes.c_void_p)), "debug_stage")
         return out.reshape(3 * nrhs, inf.K, inf.K, inf.RS)
+
+
+def _tiles(d):
+    if d is None:
+        return None
+    t = SpmeTiles()
+    for k, v in d.items():
+        setattr(t, k, int(v))
+    return t
+
+
+class Group:
+    """Independent systems applied together: one launch per stage over all
+    of them, optionally replayed as one captured hipGraph.  The engines stay
+    owned by the caller and must outlive the group."""
+
+    def __init__(self, engines, nrhs):
+        self._lib = load()
+        self.engines = list(engines)
+        self.nrhs = int(nrhs)
+        arr = (ctypes.c_void_p * len(self.engines))(*[e._h.value for e in self.engines])
+        h = ctypes.c_void_p()
+        check(self._lib.spme_group_create(len(self.engines), arr, self.nrhs,
+                                          ctypes.byref(h)), "spme_group_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.spme_group_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tiles(self, tiles=None):
+        t = _tiles(tiles)
+        check(self._lib.spme_group_set_tiles(
+            self._h, ctypes.byref(t) if t is not None else None), "group set_tiles")
+
+    def tiles(self):
+        t = SpmeTiles()
+        check(self._lib.spme_group_get_tiles(self._h, ctypes.byref(t)), "group get_tiles")
+        return t.as_dict()
+
+    def autotune(self, parts=SPME_RECIP, reps=5, stream=None):
+        torch = _torch()
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        t = SpmeTiles()
+        check(self._lib.spme_group_autotune(self._h, int(parts), int(reps),
+                                            ctypes.byref(t), ctypes.c_void_p(s)),
+              "group autotune")
+        return t.as_dict()
+
+    def apply(self, fs, us, alpha=1.0, beta=0.0, parts=SPME_RECIP, graph=True,
+              stream=None):
+        """us[s] <- beta*us[s] + alpha*M_s fs[s]; fs[s], us[s]: torch CUDA
+        tensors [nrhs, 3 N_s] (last dim contiguous)."""
+        torch = _torch()
+        n = len(self.engines)
+        assert len(fs) == n and len(us) == n
+        vp, lg = ctypes.c_void_p, ctypes.c_long
+        fin = (vp * n)(*[f.data_ptr() for f in fs])
+        fout = (vp * n)(*[u.data_ptr() for u in us])
+        ldi = (lg * n)(*[f.stride(0) for f in fs])
+        ldo = (lg * n)(*[u.stride(0) for u in us])
+        for f, u, e in zip(fs, us, self.engines):
+            assert f.dtype == e.dtype and u.dtype == e.dtype
+            assert f.stride(1) == 1 and u.stride(1) == 1 and f.shape[0] == self.nrhs
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(self._lib.spme_group_apply(self._h, float(alpha), fin, ldi, float(beta), fout,
+                                         ldo, int(parts), int(bool(graph)), vp(s)),
+              "group apply")
+        return us
 
 
 class DropIn:

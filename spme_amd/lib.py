@@ -22,7 +22,13 @@ EXPORTED = [
     "spme_engine_set_positions", "spme_engine_apply", "spme_engine_info",
     "spme_engine_debug_stage", "spme_engine_set_timing", "spme_engine_get_timing",
     "spme_mlp_nparam", "spme_mlp_forward", "spme_mlp_backward", "spme_mlp_adam",
+    "spme_engine_set_tiles", "spme_engine_get_tiles", "spme_engine_autotune",
+    "spme_group_create", "spme_group_destroy", "spme_group_set_tiles",
+    "spme_group_get_tiles", "spme_group_autotune", "spme_group_apply",
 ]
+
+TILE_FIELDS = ["spread_cc", "spread_zc", "interp_zc", "spread_persist",
+               "interp_persist", "real_persist"]
 
 STAGES = ["permute", "spread", "fft_x_fwd", "fft_y_fwd", "fft_z_infl",
           "fft_y_inv", "fft_x_inv", "interp", "permute_real", "real"]
@@ -42,6 +48,13 @@ class SpmeInfo(ctypes.Structure):
                 ("L", ctypes.c_double), ("xi", ctypes.c_double),
                 ("rc", ctypes.c_double), ("mesh_bytes", ctypes.c_long),
                 ("stream", ctypes.c_void_p)]
+
+
+class SpmeTiles(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in TILE_FIELDS]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n in TILE_FIELDS}
 
 
 class SpmeError(RuntimeError):
@@ -87,6 +100,17 @@ def load() -> ctypes.CDLL:
     L.spme_mlp_forward.argtypes = [vp, vp, i, vp, vp, vp, vp]
     L.spme_mlp_backward.argtypes = [vp, vp, vp, vp, vp, i, vp, vp, vp, vp, vp]
     L.spme_mlp_adam.argtypes = [vp, vp, vp, vp, f, f, f, f, i, vp]
+    tp = ctypes.POINTER(SpmeTiles)
+    L.spme_engine_set_tiles.argtypes = [vp, tp]
+    L.spme_engine_get_tiles.argtypes = [vp, i, tp]
+    L.spme_engine_autotune.argtypes = [vp, i, i, tp, vp]
+    L.spme_group_create.argtypes = [i, ctypes.POINTER(vp), i, ctypes.POINTER(vp)]
+    L.spme_group_destroy.argtypes = [vp]
+    L.spme_group_set_tiles.argtypes = [vp, tp]
+    L.spme_group_get_tiles.argtypes = [vp, tp]
+    L.spme_group_autotune.argtypes = [vp, i, i, tp, vp]
+    L.spme_group_apply.argtypes = [vp, d, ctypes.POINTER(vp), ctypes.POINTER(l), d,
+                                   ctypes.POINTER(vp), ctypes.POINTER(l), i, i, vp]
     _lib = L
     return L
 

@@ -106,4 +106,42 @@ CONFIGS = {
     "c2": dict(seed=1002, n=100_000, phi=0.2, K=128, p=6, rc=4.0, nrhs=1),
     # batched: N=200k, phi=0.2, K=192, p=8, rc=4, 16 RHS
     "c3": dict(seed=1003, n=200_000, phi=0.2, K=192, p=8, rc=4.0, nrhs=16),
+    # uneven systems: 64 boxes, N in [2e3, 5e4], phi in [0.05, 0.3], K in
+    # {32, 48, 64, 96}, p=6, 4 RHS, FP32 (multi_systems below)
+    "c4": dict(seed=1004, nsys=64, p=6, nrhs=4),
 }
+
+
+@dataclass
+class GroupMember:
+    """One system of an uneven multi-system workload, with its parameters."""
+    seed: int
+    n: int
+    phi: float
+    L: float
+    K: int
+    p: int
+    xi: float
+    rc: float
+
+
+def multi_systems(seed: int = 1004, nsys: int = 64, n_range=(2000, 50000),
+                  phi_range=(0.05, 0.3), meshes=(32, 48, 64, 96), h_max: float = 1.2,
+                  p: int = 6, tol: float = 1e-4, nrhs: int = 4):
+    """Independent periodic boxes: N log-uniform in n_range, phi uniform in
+    phi_range, mesh the smallest of `meshes` with L/K <= h_max (else the
+    largest), B-spline order p; xi and rc from the analytic estimate for the
+    relative error tol at that mesh.  System s draws its sizes from stream 3
+    of `seed` and its positions and forces from seed*1000 + s."""
+    from . import params as PR
+    u = uniform(seed, 3, 2 * nsys)
+    lo, hi = math.log(n_range[0]), math.log(n_range[1])
+    out = []
+    for s in range(nsys):
+        n = int(math.exp(lo + u[2 * s] * (hi - lo)))
+        phi = phi_range[0] + (phi_range[1] - phi_range[0]) * float(u[2 * s + 1])
+        L = box_length(n, phi)
+        K = next((k for k in meshes if L / k <= h_max), meshes[-1])
+        a = PR.analytic_select(n, phi, tol, nrhs=nrhs, fixed_K=K, fixed_p=p)
+        out.append(GroupMember(seed * 1000 + s, n, phi, L, K, p, a.xi, a.rc))
+    return out
