@@ -82,7 +82,7 @@ int spme_engine_get_timing(spme_t *spme, double *ms_total, int *launches);
  *   spread_cc   components per spreading pass: 3, 6 or 12; ignored when it
  *               does not divide 3*nrhs
  *   spread_zc   z planes per spreading block, 1..256 (clamped to K)
- *   interp_zc   z planes per interpolation block, 1..64 (clamped to K)
+ *   interp_zc   z planes per interpolation block, 1..256 (clamped to K)
  *   *_persist   grouped launches only (spme_group_*): resident blocks per
  *               compute unit of a persistent launch of that stage, 1..64;
  *               0 launches one block per tile. */

@@ -167,14 +167,75 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
 // ---------------------------------------------------------------------------
 // Interpolation (old-version/spme_kernel.c:8-70):
 //   out[r*ldout + 3*i + d] = beta*out + alpha * sum_j q_i(j) grid[3r+d](j).
-// A block owns a TXY x TXY tile of stencil bases over a z range and a chunk
-// of CC components.  It streams the tile plus its (P-1)-wide halo through
-// LDS one z plane at a time (P+1 slots: the P planes the current particles
-// need and the one being filled); each particle is handled by exactly one
-// wave (lanes = the P x P stencil columns) and written without atomics.
-// The loads of step s+1 (its new plane and its particles) are issued before
-// step s is processed, so each step costs one barrier and no exposed
-// global-memory latency.
+// A block owns a TXY x TXY tile of stencil bases, a z range and a chunk of
+// CC components (CC*sizeof(T) = 16 bytes: 4 FP32 or 2 FP64).  The tile and
+// its (P-1)-wide halo are staged in LDS with the CC components of a point
+// side by side, so one 16-byte LDS read gives a lane all its components.
+// The z range is swept in phases of S steps: a ring of S + P - 1 planes
+// holds what the particles of one phase need, and the S new planes and the
+// particles of the next phase are loaded into registers while the current
+// phase is processed (two barriers per phase).  Each particle is handled by
+// one wave, lanes = the P x P stencil columns; the per-lane products are
+// reduced over the wave for all CC components at once and written without
+// atomics.  Particles of a phase are the contiguous slot ranges of the
+// 8 x 8 sort tiles inside the block tile.
+
+// Wave-wide sums of CC values per lane (DPP lane moves and two cross-row
+// shuffles).  Returns, in every lane, the total of component (lane % CC).
+// dpp_ctrl: quad_perm [1,0,3,2] = 0xB1, [2,3,0,1] = 0x4E, row_ror:4 =
+// 0x124, row_ror:8 = 0x128.
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, int(b), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+template <typename T>
+__device__ __forceinline__ T sum_same_residue4(T c) {
+  // lanes i, i^4, i^8, ... (all lanes with equal i % 4)
+  c += dpp_mov<0x124>(c);
+  c += dpp_mov<0x128>(c);
+  c += __shfl_xor(c, 16);
+  c += __shfl_xor(c, 32);
+  return c;
+}
+
+template <typename T, int CC>
+__device__ __forceinline__ T reduce_components(const T (&v)[CC], int lane) {
+  static_assert(CC == 4 || CC == 2, "4 FP32 or 2 FP64 components");
+  const bool o = lane & 1;
+  if constexpr (CC == 4) {
+    // round 1 (partner lane ^ 1): even lanes keep components 0, 2, odd 1, 3
+    const T a = (o ? v[1] : v[0]) + dpp_mov<0xB1>(o ? v[0] : v[1]);
+    const T b = (o ? v[3] : v[2]) + dpp_mov<0xB1>(o ? v[2] : v[3]);
+    // round 2 (partner lane ^ 2): component lane % 4
+    const bool h = lane & 2;
+    const T c = (h ? b : a) + dpp_mov<0x4E>(h ? a : b);
+    return sum_same_residue4(c);
+  } else {
+    T c = (o ? v[1] : v[0]) + dpp_mov<0xB1>(o ? v[0] : v[1]);
+    c += dpp_mov<0x4E>(c);  // lanes with equal parity of the quad
+    return sum_same_residue4(c);
+  }
+}
+
+template <typename T> struct alignas(16) Vec16 {
+  static constexpr int N = 16 / sizeof(T);
+  T v[N];
+};
+
+// Steps per phase and particles staged per batch.
+template <int P> struct InterpPhase {
+  static constexpr int S = 8;
+  static constexpr int MAXB = 128;
+};
+
 template <typename T, int P, int CC, int TXY, int NT>
 __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
                                              const int *__restrict__ pbase,
@@ -183,100 +244,108 @@ __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
                                              const T *__restrict__ grid, long comp_stride,
                                              int RS, int K, int C, int ZC, T alpha, T beta,
                                              T *__restrict__ out, long ldout, int lb) {
+  static_assert(CC * sizeof(T) == 16, "16-byte component groups");
+  using V = Vec16<T>;
+  constexpr int S = InterpPhase<P>::S, MAXB = InterpPhase<P>::MAXB;
   constexpr int E = TXY + P - 1;
-  constexpr int EX = E | 1;  // odd row pitch
-  constexpr int PLANE = CC * E * EX;
-  constexpr int NSLOT = P + 1;
-  constexpr int NSUB = TXY / 8;  // sort tiles are 8 x 8
+  constexpr int PLANE = E * E;          // points per plane
+  constexpr int NSLOT = S + P - 1;      // ring of planes
+  constexpr int NSUB = TXY / 8;         // sort tiles are 8 x 8
   constexpr int NS2 = NSUB * NSUB;
-  constexpr int MAXST = 64;  // >= ZC
   constexpr int NWV = NT / 64;
-  constexpr int MAXB = 32;  // particles staged per batch
   constexpr int NW = 3 * P;
-  constexpr int NPL = (CC * E * E + NT - 1) / NT;  // plane values per thread
-  constexpr int RW = (MAXB * NW + NT - 1) / NT;    // weights per thread
-  __shared__ T win[NSLOT * PLANE];
-  __shared__ int s_lo[NS2 * MAXST];
-  __shared__ int s_n[NS2 * MAXST];
-  __shared__ int s_pb[2][MAXB], s_perm[2][MAXB];
-  __shared__ T s_w[2][MAXB * NW];
+  constexpr int NPT = (S * PLANE + NT - 1) / NT;   // plane points per thread per phase
+  constexpr int RW = (MAXB * NW + NT - 1) / NT;    // weights per thread per batch
+  __shared__ V win[NSLOT * PLANE];
+  __shared__ int s_pb[MAXB], s_perm[MAXB];
+  __shared__ __attribute__((aligned(16))) T s_w[MAXB * NW];
+
   const int nt = K / TXY, nz = (K + ZC - 1) / ZC;
-  // logical block: tiles fastest, so each XCD gets a band of adjacent tiles
   const int tile = lb % (nt * nt), rest = lb / (nt * nt);
   const int tx = tile % nt, ty = tile / nt;
   const int x0 = tx * TXY, y0 = ty * TXY;
   const int z0 = (rest % nz) * ZC, zend = min(z0 + ZC, K);
-  const int nst = zend - z0;
   const int cc0 = (rest / nz) * CC;
+  const int ncc = min(CC, C - cc0);  // valid components of this chunk
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const T *g0 = grid + (size_t)cc0 * comp_stride;
-
-  for (int q = tid; q < NS2 * nst; q += NT) {
-    const int sq = q / nst, st = q - sq * nst;
-    const int sx = tx * NSUB + sq % NSUB, sy = ty * NSUB + sq / NSUB;
-    const int key = (sy * (K / 8) + sx) * K + z0 + st;
-    const int lo = boff[key];
-    s_lo[q] = lo;
-    s_n[q] = boff[key + 1] - lo;
-  }
-  // per-thread plane offsets, fixed over the sweep: source offset within a
-  // mesh z plane (components of the chunk included) and LDS offset within a
-  // window slot; -1 past the plane
-  long src_off[NPL];
-  int dst_off[NPL];
+  const long zstride = (long)K * RS;
+  // key of sort tile q at base z: ((sy*(K/8) + sx)*K + z)
+  int key0[NS2];
 #pragma unroll
-  for (int r = 0; r < NPL; ++r) {
-    const int q = tid + NT * r;
-    const int c = q / (E * E), rr = q - c * (E * E);
-    const int ey = rr / E, ex = rr - ey * E;
+  for (int q = 0; q < NS2; ++q) {
+    const int sx = tx * NSUB + q % NSUB, sy = ty * NSUB + q / NSUB;
+    key0[q] = (sy * (K / 8) + sx) * K;
+  }
+
+  // plane point p of the tile (p < PLANE): source offset in a mesh plane
+  auto src_of = [&](int p) {
+    const int ey = p / E, ex = p - ey * E;
     int yy = y0 + ey, xx = x0 + ex;
     yy -= yy >= K ? K : 0;
     xx -= xx >= K ? K : 0;
-    src_off[r] = (long)c * comp_stride + (long)yy * RS + xx;
-    dst_off[r] = q < CC * E * E ? (c * E + ey) * EX + ex : -1;
-  }
-  const long zstride = (long)K * RS;
-  for (int j = 0; j < P - 1; ++j) {
-    const T *gz = g0 + ((z0 + j) % K) * zstride;
-    T *wz = win + ((z0 + j) % NSLOT) * PLANE;
+    return (long)yy * RS + xx;
+  };
+  auto load_point = [&](int z, int p) {  // z unwrapped
+    V v;
+    const T *g = g0 + (z >= K ? z - K : z) * zstride + src_of(p);
 #pragma unroll
-    for (int r = 0; r < NPL; ++r)
-      if (dst_off[r] >= 0) wz[dst_off[r]] = gz[src_off[r]];
+    for (int c = 0; c < CC; ++c) v.v[c] = c < ncc ? g[(size_t)c * comp_stride] : T(0);
+    return v;
+  };
+  // prologue: planes z0 .. z0+P-2
+  for (int q = tid; q < (P - 1) * PLANE; q += NT) {
+    const int j = q / PLANE, p = q - j * PLANE;
+    win[((z0 + j) % NSLOT) * PLANE + p] = load_point(z0 + j, p);
   }
-  __syncthreads();
 
-  // registers carrying the next step's loads
-  T vp[NPL], vw[RW];
-  int vpb = 0, vpm = 0, nb_next = 0;
-  // issue the loads of step st, particles [b0, b0 + MAXB) of it
-  auto fetch = [&](int st, int b0, bool with_plane) {
-    int cnt[NS2], lo[NS2];
-    int n = 0;
+  // registers carrying the next phase's loads
+  V vp[NPT];
+  T vw[RW];
+  int vpb = 0, vpm = 0;
+  int lo[NS2], cnt[NS2];
+  int n_all = 0;
+  // phase starting at step zs: the new planes zs+P-1 .. zs+P-1+S-1 (those
+  // below zend+P-1) and the particle ranges; loads of batch b0
+  auto ranges = [&](int zs) {
+    const int ze = min(zs + S, zend);
+    n_all = 0;
 #pragma unroll
     for (int q = 0; q < NS2; ++q) {
-      cnt[q] = s_n[q * nst + st];
-      lo[q] = s_lo[q * nst + st];
-      n += cnt[q];
+      lo[q] = boff[key0[q] + zs];
+      cnt[q] = boff[key0[q] + ze] - lo[q];
+      n_all += cnt[q];
     }
-    auto locate = [&](int k) {
-      int i = lo[0] + k;
+  };
+  auto locate = [&](int k) {  // k-th particle of the phase -> sorted slot
+    int i = lo[0] + k, rem = k - cnt[0];
 #pragma unroll
-      for (int q = 0; q < NS2 - 1; ++q) {
-        if (k >= cnt[q]) {
-          k -= cnt[q];
-          i = lo[q + 1] + k;
-        } else {
-          break;
-        }
-      }
-      return i;
-    };
-    const int nb = max(0, min(MAXB, n - b0));
-    if (with_plane) {
-      const T *gz = g0 + ((z0 + st + P - 1) % K) * zstride;
-#pragma unroll
-      for (int r = 0; r < NPL; ++r) vp[r] = dst_off[r] >= 0 ? gz[src_off[r]] : T(0);
+    for (int q = 1; q < NS2; ++q) {
+      i = rem >= 0 ? lo[q] + rem : i;
+      rem -= cnt[q];
     }
+    return i;
+  };
+  auto fetch_planes = [&](int zs) {
+    const int zfirst = zs + P - 1, zlast = min(zs + S, zend) + P - 1;  // [zfirst, zlast)
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+      const int q = tid + NT * r;
+      const int j = q / PLANE, p = q - j * PLANE;
+      if (zfirst + j < zlast && q < S * PLANE) vp[r] = load_point(zfirst + j, p);
+    }
+  };
+  auto commit_planes = [&](int zs) {
+    const int zfirst = zs + P - 1, zlast = min(zs + S, zend) + P - 1;
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+      const int q = tid + NT * r;
+      const int j = q / PLANE, p = q - j * PLANE;
+      if (zfirst + j < zlast && q < S * PLANE) win[((zfirst + j) % NSLOT) * PLANE + p] = vp[r];
+    }
+  };
+  auto fetch_particles = [&](int b0) {
+    const int nb = max(0, min(MAXB, n_all - b0));
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
       const int g = tid + NT * r;
@@ -287,74 +356,84 @@ __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
       vpb = pbase[i];
       vpm = perm[i];
     }
-    nb_next = nb;
-    return n;
+    return nb;
   };
-  auto commit = [&](int st, int buf, bool with_plane) {
-    if (with_plane) {
-      T *wz = win + ((z0 + st + P - 1) % NSLOT) * PLANE;
-#pragma unroll
-      for (int r = 0; r < NPL; ++r)
-        if (dst_off[r] >= 0) wz[dst_off[r]] = vp[r];
-    }
+  auto commit_particles = [&]() {
 #pragma unroll
     for (int r = 0; r < RW; ++r)
-      if (tid + NT * r < MAXB * NW) s_w[buf][tid + NT * r] = vw[r];
+      if (tid + NT * r < MAXB * NW) s_w[tid + NT * r] = vw[r];
     if (tid < MAXB) {
-      s_pb[buf][tid] = vpb;
-      s_perm[buf][tid] = vpm;
+      s_pb[tid] = vpb;
+      s_perm[tid] = vpm;
     }
   };
 
-  const int dx = lane % P, dy = lane / P;
-  const bool act = lane < P * P;
-  int n_cur = fetch(0, 0, true);
-  int b_cur = 0, st = 0, buf = 0;
-  while (st < nst) {
-    const int bz = z0 + st;
-    const int slot0 = bz % NSLOT;
-    const int nb = nb_next;
-    commit(st, buf, b_cur == 0);
-    __syncthreads();
-    // next unit of work: the rest of this step, or the next step
-    const int b_nxt = b_cur + MAXB < n_cur ? b_cur + MAXB : 0;
-    const int st_nxt = b_nxt ? st : st + 1;
-    int n_nxt = n_cur;
-    if (st_nxt < nst) n_nxt = fetch(st_nxt, b_nxt, b_nxt == 0);
+  const int dx = lane & 7, dy = lane >> 3;
+  const bool act = dx < P && dy < P;
+  const int dxc = min(dx, P - 1), dyc = min(dy, P - 1);
+  auto process = [&](int nb) {
     for (int k = wave; k < nb; k += NWV) {
-      const int pb = s_pb[buf][k];
+      const int pb = s_pb[k];
       const int rx = (pb & 1023) - x0, ry = ((pb >> 10) & 1023) - y0;
-      const T *w = s_w[buf] + k * NW;
-      const T wxy = act ? w[dx] * w[P + dy] : T(0);
+      const int bz = pb >> 20;  // in [z0, zend)
+      const T *w = s_w + k * NW;
+      const T wxy = act ? w[dxc] * w[P + dyc] : T(0);
       T acc[CC];
 #pragma unroll
       for (int c = 0; c < CC; ++c) acc[c] = T(0);
-      if (act) {
-        const int off = (ry + dy) * EX + rx + dx;
+      int sl = bz % NSLOT;
+      const V *pt = win + (ry + dyc) * E + rx + dxc;
 #pragma unroll
-        for (int j = 0; j < P; ++j) {
-          const T wz = w[2 * P + j];
-          const int sl = slot0 + j >= NSLOT ? slot0 + j - NSLOT : slot0 + j;
-          const T *pl = win + sl * PLANE + off;
+      for (int j = 0; j < P; ++j) {
+        const T wz = w[2 * P + j];
+        const V v = pt[sl * PLANE];
 #pragma unroll
-          for (int c = 0; c < CC; ++c) acc[c] += wz * pl[c * E * EX];
-        }
+        for (int c = 0; c < CC; ++c) acc[c] += wz * v.v[c];
+        sl = sl + 1 == NSLOT ? 0 : sl + 1;
       }
-      const int o = s_perm[buf][k];
 #pragma unroll
-      for (int c = 0; c < CC; ++c) {
-        const T v = wave_sum(acc[c] * wxy);
-        if (lane == 0) {
-          const int comp = cc0 + c, r = comp / 3, d = comp - 3 * r;
-          T *u = out + r * ldout + 3L * o + d;
-          *u = beta == T(0) ? alpha * v : beta * *u + alpha * v;
-        }
+      for (int c = 0; c < CC; ++c) acc[c] *= wxy;
+      const T tot = reduce_components<T, CC>(acc, lane);
+      if (lane < ncc) {
+        const int comp = cc0 + lane, r = comp / 3, d = comp - 3 * r;
+        T *u = out + r * ldout + 3L * s_perm[k] + d;
+        *u = beta == T(0) ? alpha * tot : beta * *u + alpha * tot;
       }
     }
-    b_cur = b_nxt;
-    st = st_nxt;
-    n_cur = n_nxt;
-    buf ^= 1;
+  };
+
+  ranges(z0);
+  fetch_planes(z0);
+  int nb = fetch_particles(0);
+  for (int zs = z0; zs < zend; zs += S) {
+    const int n_phase = n_all;
+    commit_planes(zs);
+    commit_particles();
+    __syncthreads();
+    const int zn = zs + S;
+    int nb_next = 0;
+    if (n_phase <= MAXB && zn < zend) {
+      ranges(zn);
+      fetch_planes(zn);
+      nb_next = fetch_particles(0);
+    }
+    process(nb);
+    if (n_phase > MAXB) {  // rare: the phase holds more particles than a batch
+      for (int b0 = MAXB; b0 < n_phase; b0 += MAXB) {
+        __syncthreads();
+        const int nbb = fetch_particles(b0);
+        commit_particles();
+        __syncthreads();
+        process(nbb);
+      }
+      if (zn < zend) {
+        ranges(zn);
+        fetch_planes(zn);
+        nb_next = fetch_particles(0);
+      }
+    }
+    __syncthreads();
+    nb = nb_next;
   }
 }
 

@@ -410,8 +410,7 @@ Tiles resolve_tiles(const _spme_t *e, const spme_tiles_t &t, int C) {
   Tiles r;
   r.spread_cc = (t.spread_cc > 0 && C % t.spread_cc == 0) ? t.spread_cc : pick_cc(C, e->tsz);
   r.spread_zc = t.spread_zc > 0 ? std::min(t.spread_zc, K) : (K >= 64 ? K / 4 : K / 2);
-  const int izmax = std::min(K, InterpShape<float>::MAXZC);
-  r.interp_zc = t.interp_zc > 0 ? std::min(t.interp_zc, izmax) : std::min(K >= 64 ? K / 4 : K, izmax);
+  r.interp_zc = t.interp_zc > 0 ? std::min(t.interp_zc, K) : (K >= 128 ? K / 2 : K);
   return r;
 }
 
@@ -588,7 +587,7 @@ int autotune_t(_spme_t *e, int nrhs, int reps, hipStream_t st) {
     }
   }
   best_ms = 1e30f;
-  for (int zc : zc_candidates(K, 4, std::min(K, (int)InterpShape<T>::MAXZC))) {
+  for (int zc : zc_candidates(K, 8, std::min(K, (int)InterpShape<T>::MAXZC))) {
     InterpArgs<T> ia{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid,
                      e->comp_stride, e->RS, K, N, C, e->P, zc, T(1), T(0), (T *)out.p,
                      3L * N};
@@ -967,8 +966,8 @@ int group_autotune_t(_spme_group_t *g, int parts, int reps, hipStream_t st) {
     }
   }
   best_ms = 1e30f;
-  for (int zc : {0, 8, 16, 24, 32, 48, 64}) {
-    for (int ps : {0, 1, 2, 4}) {
+  for (int zc : {0, 16, 32, 48, 64, 96}) {
+    for (int ps : {0, 1, 2}) {
       spme_tiles_t c = best;
       c.interp_zc = zc;
       c.interp_persist = ps;

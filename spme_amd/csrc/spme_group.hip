@@ -259,7 +259,7 @@ int launch_group_fft(const GroupStage<FftArgs<T>> &t, int pass, int kmax, hipStr
 
 template <typename T, int P, int TXY>
 static void group_interp_t(const GroupStage<InterpArgs<T>> &t, int grid, hipStream_t st) {
-  constexpr int NT = 512, CC = InterpShape<T>::CC;
+  constexpr int NT = InterpShape<T>::NT, CC = InterpShape<T>::CC;
   if (grid < t.total)
     hipLaunchKernelGGL((kg_interp<T, P, CC, TXY, NT, true>), dim3(grid), dim3(NT), 0, st, t);
   else

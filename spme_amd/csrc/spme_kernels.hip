@@ -306,11 +306,11 @@ int launch_spread(const SpreadArgs<T> &a, hipStream_t st) {
   return -1;
 }
 
-// Interpolation shape: FP32 takes 3 components per block over a 16 x 16
-// tile; FP64 takes 1 (LDS: P planes x CC x (TXY+P-1)^2 values).
+// Interpolation shape: 16 bytes of components (4 FP32, 2 FP64) per block
+// over a 16 x 16 tile (8 x 8 when K is not a multiple of 16).
 template <typename T, int P, int CC, int TXY>
 static void interp_cc(const InterpArgs<T> &a, hipStream_t st) {
-  constexpr int NT = 512;
+  constexpr int NT = InterpShape<T>::NT;
   const int nblk = interp_blocks(a.K, a.ZC, a.C, CC, TXY);
   hipLaunchKernelGGL((k_interp<T, P, CC, TXY, NT>), dim3(nblk), dim3(NT), 0, st, a.boff, a.pbase,
                      a.pw, a.perm, a.grid, a.comp_stride, a.RS, a.K, a.C, a.ZC, a.alpha, a.beta,
@@ -320,7 +320,7 @@ static void interp_cc(const InterpArgs<T> &a, hipStream_t st) {
 template <typename T, int P>
 static int interp_p(const InterpArgs<T> &a, hipStream_t st) {
   constexpr int CC = InterpShape<T>::CC;
-  if (a.C % CC || a.ZC < 1 || a.ZC > InterpShape<T>::MAXZC) return -1;
+  if (a.ZC < 1 || a.ZC > InterpShape<T>::MAXZC) return -1;
   if (a.K % 16 == 0) interp_cc<T, P, CC, 16>(a, st);
   else interp_cc<T, P, CC, 8>(a, st);
   return 0;

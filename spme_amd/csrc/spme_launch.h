@@ -40,14 +40,15 @@ constexpr int SPREAD_MAXZC = 256;
 inline int spread_blocks(int K, int ZC, int C, int CC) {
   return (K / 8) * (K / 8) * ((K + ZC - 1) / ZC) * (C / CC);
 }
-// Interpolation: CC components per block over a TXY x TXY tile, z chunk of
-// ZC <= MAXZC planes (LDS bucket table).
+// Interpolation: a chunk of CC components (16 bytes, the last chunk may be
+// partial) per block over a TXY x TXY tile and a z chunk of ZC planes.
 template <typename T> struct InterpShape {
-  static constexpr int CC = sizeof(T) == 4 ? 3 : 1;
-  static constexpr int MAXZC = 64;
+  static constexpr int CC = 16 / sizeof(T);
+  static constexpr int MAXZC = 256;
+  static constexpr int NT = 1024;
 };
 inline int interp_blocks(int K, int ZC, int C, int CC, int TXY) {
-  return (K / TXY) * (K / TXY) * ((K + ZC - 1) / ZC) * (C / CC);
+  return (K / TXY) * (K / TXY) * ((K + ZC - 1) / ZC) * ((C + CC - 1) / CC);
 }
 // FFT line batches: keep ~24 KB of LDS per block.
 template <typename T> struct FftShape {

@@ -196,5 +196,5 @@ def test_group_errors():
     with pytest.raises(SpmeError, match="spread_cc"):
         g.set_tiles(dict(spread_cc=5))
     with pytest.raises(SpmeError, match="interp_zc"):
-        g.set_tiles(dict(interp_zc=65))
+        g.set_tiles(dict(interp_zc=257))
     g.close()
