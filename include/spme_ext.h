@@ -110,8 +110,10 @@ int spme_engine_autotune(spme_t *spme, int nrhs, int reps, spme_tiles_t *best, v
  * with graph != 0 the whole apply is one captured hipGraph, replayed on
  * later calls.  The engines must have positions set, one precision and one
  * B-spline order, and max_rhs >= nrhs; they stay owned by the caller and
- * must outlive the group.  A group gives bitwise the result of the separate
- * spme_engine_apply calls.  It uses the engines' meshes and buffers: do not
+ * must outlive the group.  For members with K <= 96 a group gives bitwise
+ * the result of the separate spme_engine_apply calls; larger members use
+ * separate x and y FFT passes where a single apply fuses them, so their
+ * results agree to rounding.  It uses the engines' meshes and buffers: do not
  * apply a member engine on another stream while a group apply may run.
  * Changing an engine's positions is allowed (the group notices). */
 typedef struct _spme_group_t spme_group_t;

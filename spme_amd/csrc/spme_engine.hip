@@ -161,6 +161,7 @@ struct _spme_t {
   // that groups holding this engine notice a rebuild of its buffers
   spme_tiles_t tiles;
   unsigned pos_version;
+  int ncu;  // compute units of the device
 };
 
 namespace {
@@ -255,6 +256,7 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
       return bail(fail(-2, "%s failed: %s", #expr, hipGetErrorString(e_)));              \
   } while (0)
   E_TRY(hipGetDevice(&e->device));
+  E_TRY(hipDeviceGetAttribute(&e->ncu, hipDeviceAttributeMultiprocessorCount, e->device));
   E_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   E_TRY(hipMalloc(&e->grid, (size_t)C * e->comp_stride * e->tsz));
   int rc = e->prec == SPME_FP32 ? upload_tables<float>(e) : upload_tables<double>(e);
@@ -458,14 +460,16 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
     mark(SPME_ST_SPREAD);
     if (stop_after == 1) return 0;
     FftArgs<T> fa{(T *)e->grid, e->comp_stride, e->RS, e->K, C, (const T *)e->tab,
-                  T(2.0 * M_PI / e->prm.box_size), (const cpx<T> *)e->tw};
-    hipEvent_t fev[5];
+                  T(2.0 * M_PI / e->prm.box_size), (const cpx<T> *)e->tw, e->ncu};
+    FftMarks fm{};
     if (timed)
-      for (int i = 0; i < 5; ++i) fev[i] = take_event(e);
-    if (launch_fft_influence<T>(fa, st, timed ? fev : nullptr))
+      for (int i = 0; i < 5; ++i) fm.ev[i] = take_event(e);
+    if (launch_fft_influence<T>(fa, st, timed ? &fm : nullptr))
       return fail(-1, "fft: unsupported size");
-    if (timed)
-      for (int i = 0; i < 5; ++i) rec.marks.push_back({SPME_ST_FFT_X_FWD + i, fev[i]});
+    if (timed) {
+      for (int i = 0; i < fm.n; ++i) rec.marks.push_back({SPME_ST_FFT_X_FWD + fm.pass[i], fm.ev[i]});
+      for (int i = fm.n; i < 5; ++i) e->pool.push_back(fm.ev[i]);
+    }
     if (stop_after == 2) return 0;
     InterpArgs<T> ia{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid,
                      e->comp_stride, e->RS, e->K, N, C, e->P, tl.interp_zc, T(alpha), T(beta),

@@ -9,6 +9,8 @@
 // z of the stencil base); bucket k holds sorted slots [boff[k], boff[k+1]).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "spme_bodies.h"
 #include "spme_kernels.h"
 #include "spme_launch.h"
@@ -196,6 +198,327 @@ __global__ __launch_bounds__(NT) void k_fft_z_infl(T *__restrict__ grid, long co
 }
 
 // ---------------------------------------------------------------------------
+// Fused x + y transforms: one (component, z) plane at a time in LDS, so the
+// real mesh is read once and the half spectrum written once per direction
+// (the separate x and y passes move the spectrum through HBM twice more).
+// LDS holds K/2 line pairs of pitch LP = 2*Kh complex: during the x
+// transform line l is the complex row (row 2l) + i (row 2l+1); after the
+// split, row y of the half spectrum sits at y*Kh (2l*Kh = l*LP and
+// (2l+1)*Kh = l*LP + Kh), i.e. the plane is [y][x] with pitch Kh for the y
+// transform.  Blocks loop over planes and load the next plane into
+// registers while the second half of the current one is transformed and
+// stored.
+template <typename T, int K> struct XYShape {
+  static constexpr int Kh = K / 2 + 1;
+  static constexpr int LP = 2 * Kh;
+  static constexpr int NCPX = (K / 2) * LP;
+  static constexpr size_t LDS = (NCPX + K) * sizeof(cpx<T>);
+};
+
+// Split of packed lines into two half spectra, in place: for line l and
+// k <= K/2, A_k = (Z_k + conj Z_{K-k})/2 (row 2l), B_k = (Z_k - conj
+// Z_{K-k})/(2i) (row 2l+1).  Writing B_k (position Kh + k) clobbers Z at
+// K - (K/2 - 1 - k), so k and K/2 - 1 - k go to one thread.
+template <typename T, int K>
+__device__ __forceinline__ void xy_split(cpx<T> *line, int k) {
+  constexpr int Kh = K / 2 + 1;
+  const cpx<T> Z = line[k], Zm = cconj(line[k == 0 ? 0 : K - k]);
+  const cpx<T> A = {T(0.5) * (Z.x + Zm.x), T(0.5) * (Z.y + Zm.y)};
+  const cpx<T> D = csub(Z, Zm);
+  const cpx<T> B = {T(0.5) * D.y, T(-0.5) * D.x};
+  line[k] = A;
+  line[Kh + k] = B;
+}
+
+// Inverse of the split (packing two half spectra into one line of K),
+// dropping the imaginary parts at k = 0 and K/2 (real part of the IDFT).
+template <typename T, int K>
+__device__ __forceinline__ void xy_pack(cpx<T> *line, int k) {
+  constexpr int Kh = K / 2 + 1;
+  cpx<T> A = line[k], B = line[Kh + k];
+  if (k == 0 || k == K / 2) {
+    A.y = T(0);
+    B.y = T(0);
+  }
+  line[k] = {A.x - B.y, A.y + B.x};
+  if (k > 0 && k < K / 2) line[K - k] = {A.x + B.y, B.x - A.y};
+}
+
+template <typename T, int K, int NT>
+__global__ __launch_bounds__(NT) void k_fft_xy_fwd(T *__restrict__ grid, int RS, long comp_stride,
+                                                   int nplanes, const cpx<T> *__restrict__ tw) {
+  using S = XYShape<T, K>;
+  constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
+  constexpr int Kh = S::Kh, LP = S::LP, L2 = K / 2;
+  constexpr int G = 16 / sizeof(T);  // reals per 16-byte load
+  constexpr int UPL = K / G, NUNIT = L2 * UPL, NU = (NUNIT + NT - 1) / NT;
+  __shared__ cpx<T> buf[S::NCPX];
+  __shared__ cpx<T> s_tw[K];
+  using V = Vec16<T>;
+  load_tw<T, K>(s_tw, tw);
+  const int tid = threadIdx.x;
+  const int Khp = RS / 2;
+  V ra[NU], rb[NU];
+  auto fetch = [&](int q) {
+    const int comp = q / K, z = q - comp * K;
+    const T *base = grid + comp * comp_stride + (size_t)z * K * RS;
+#pragma unroll
+    for (int r = 0; r < NU; ++r) {
+      const int u = tid + NT * r;
+      if (u < NUNIT) {
+        const int l = u / UPL, g = u - l * UPL;
+        ra[r] = *reinterpret_cast<const V *>(base + (size_t)(2 * l) * RS + G * g);
+        rb[r] = *reinterpret_cast<const V *>(base + (size_t)(2 * l + 1) * RS + G * g);
+      }
+    }
+  };
+  auto commit = [&]() {
+#pragma unroll
+    for (int r = 0; r < NU; ++r) {
+      const int u = tid + NT * r;
+      if (u < NUNIT) {
+        const int l = u / UPL, g = u - l * UPL;
+#pragma unroll
+        for (int i = 0; i < G; ++i) buf[l * LP + G * g + i] = {ra[r].v[i], rb[r].v[i]};
+      }
+    }
+  };
+  int q = blockIdx.x;
+  if (q < nplanes) fetch(q);
+  for (; q < nplanes; q += gridDim.x) {
+    commit();
+    __syncthreads();
+    // x, step A: per line and n2, an N1-point FFT, twiddled, in place
+    for (int t = tid; t < L2 * N2; t += NT) {
+      const int l = t / N2, n2 = t - l * N2;
+      cpx<T> *ln = buf + l * LP;
+      cpx<T> a[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) a[n1] = ln[n1 * N2 + n2];
+      reg_fft<N1, -1>(a);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<-1>(s_tw, n2 * k1));
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) ln[k1 * N2 + n2] = a[k1];
+    }
+    __syncthreads();
+    // x, step B: N2-point FFTs into registers, then natural order
+    constexpr int PB = (L2 * N1 + NT - 1) / NT;
+    cpx<T> b[PB][N2];
+#pragma unroll
+    for (int u = 0; u < PB; ++u) {
+      const int t = tid + u * NT;
+      if (t < L2 * N1) {
+        const int l = t / N1, k1 = t - l * N1;
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) b[u][n2] = buf[l * LP + k1 * N2 + n2];
+        reg_fft<N2, -1>(b[u]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PB; ++u) {
+      const int t = tid + u * NT;
+      if (t < L2 * N1) {
+        const int l = t / N1, k1 = t - l * N1;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) buf[l * LP + k1 + N1 * k2] = b[u][k2];
+      }
+    }
+    // the next plane's loads, in flight through the rest of this one
+    if (q + (int)gridDim.x < nplanes) fetch(q + gridDim.x);
+    __syncthreads();
+    // split into half spectra: k and K/2-1-k per thread, K/2 alone
+    for (int t = tid; t < L2 * (K / 4 + 1); t += NT) {
+      const int l = t / (K / 4 + 1), m = t - l * (K / 4 + 1);
+      cpx<T> *ln = buf + l * LP;
+      if (m < K / 4) {
+        // both reads before either write
+        const int k0 = m, k1 = K / 2 - 1 - m;
+        const cpx<T> Z0 = ln[k0], Z0m = cconj(ln[k0 == 0 ? 0 : K - k0]);
+        const cpx<T> Z1 = ln[k1], Z1m = cconj(ln[K - k1]);
+        const cpx<T> D0 = csub(Z0, Z0m), D1 = csub(Z1, Z1m);
+        ln[k0] = {T(0.5) * (Z0.x + Z0m.x), T(0.5) * (Z0.y + Z0m.y)};
+        ln[Kh + k0] = {T(0.5) * D0.y, T(-0.5) * D0.x};
+        ln[k1] = {T(0.5) * (Z1.x + Z1m.x), T(0.5) * (Z1.y + Z1m.y)};
+        ln[Kh + k1] = {T(0.5) * D1.y, T(-0.5) * D1.x};
+      } else {
+        xy_split<T, K>(ln, K / 2);
+      }
+    }
+    __syncthreads();
+    // y, step A over the Kh columns of the [y][x] plane (pitch Kh)
+    for (int t = tid; t < Kh * N2; t += NT) {
+      const int x = t % Kh, n2 = t / Kh;
+      cpx<T> a[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) a[n1] = buf[(n1 * N2 + n2) * Kh + x];
+      reg_fft<N1, -1>(a);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<-1>(s_tw, n2 * k1));
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) buf[(k1 * N2 + n2) * Kh + x] = a[k1];
+    }
+    __syncthreads();
+    // y, step B straight to the spectrum rows
+    {
+      const int comp = q / K, z = q - comp * K;
+      cpx<T> *spec = reinterpret_cast<cpx<T> *>(grid + comp * comp_stride) + (size_t)z * K * Khp;
+      for (int t = tid; t < Kh * N1; t += NT) {
+        const int x = t % Kh, k1 = t / Kh;
+        cpx<T> bb[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) bb[n2] = buf[(k1 * N2 + n2) * Kh + x];
+        reg_fft<N2, -1>(bb);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) spec[(size_t)(k1 + N1 * k2) * Khp + x] = bb[k2];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int K, int NT>
+__global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS, long comp_stride,
+                                                   int nplanes, const cpx<T> *__restrict__ tw) {
+  using S = XYShape<T, K>;
+  constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
+  constexpr int Kh = S::Kh, LP = S::LP, L2 = K / 2;
+  constexpr int G = 16 / sizeof(cpx<T>);  // complex per 16-byte load
+  constexpr int UPR = (Kh + G - 1) / G, NUNIT = K * UPR, NU = (NUNIT + NT - 1) / NT;
+  __shared__ cpx<T> buf[S::NCPX];
+  __shared__ cpx<T> s_tw[K];
+  struct alignas(16) C16 { cpx<T> v[G]; };
+  load_tw<T, K>(s_tw, tw);
+  const int tid = threadIdx.x;
+  const int Khp = RS / 2;
+  C16 rv[NU];
+  auto fetch = [&](int q) {
+    const int comp = q / K, z = q - comp * K;
+    const cpx<T> *spec =
+        reinterpret_cast<const cpx<T> *>(grid + comp * comp_stride) + (size_t)z * K * Khp;
+#pragma unroll
+    for (int r = 0; r < NU; ++r) {
+      const int u = tid + NT * r;
+      if (u < NUNIT) {
+        const int y = u / UPR, g = u - y * UPR;
+        rv[r] = *reinterpret_cast<const C16 *>(spec + (size_t)y * Khp + G * g);
+      }
+    }
+  };
+  auto commit = [&]() {
+#pragma unroll
+    for (int r = 0; r < NU; ++r) {
+      const int u = tid + NT * r;
+      if (u < NUNIT) {
+        const int y = u / UPR, g = u - y * UPR;
+#pragma unroll
+        for (int i = 0; i < G; ++i)
+          if (G * g + i < Kh) buf[y * Kh + G * g + i] = rv[r].v[i];
+      }
+    }
+  };
+  int q = blockIdx.x;
+  if (q < nplanes) fetch(q);
+  for (; q < nplanes; q += gridDim.x) {
+    commit();
+    __syncthreads();
+    // y inverse, step A in place
+    for (int t = tid; t < Kh * N2; t += NT) {
+      const int x = t % Kh, n2 = t / Kh;
+      cpx<T> a[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) a[n1] = buf[(n1 * N2 + n2) * Kh + x];
+      reg_fft<N1, +1>(a);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<+1>(s_tw, n2 * k1));
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) buf[(k1 * N2 + n2) * Kh + x] = a[k1];
+    }
+    __syncthreads();
+    // y inverse, step B into registers, then natural order
+    constexpr int PY = (Kh * N1 + NT - 1) / NT;
+    cpx<T> c[PY][N2];
+#pragma unroll
+    for (int u = 0; u < PY; ++u) {
+      const int t = tid + u * NT;
+      if (t < Kh * N1) {
+        const int x = t % Kh, k1 = t / Kh;
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) c[u][n2] = buf[(k1 * N2 + n2) * Kh + x];
+        reg_fft<N2, +1>(c[u]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < PY; ++u) {
+      const int t = tid + u * NT;
+      if (t < Kh * N1) {
+        const int x = t % Kh, k1 = t / Kh;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) buf[(k1 + N1 * k2) * Kh + x] = c[u][k2];
+      }
+    }
+    // the next plane's loads, in flight through the rest of this one
+    if (q + (int)gridDim.x < nplanes) fetch(q + gridDim.x);
+    __syncthreads();
+    // pack row pairs into lines: k and K/2-1-k per thread, K/2 alone
+    for (int t = tid; t < L2 * (K / 4 + 1); t += NT) {
+      const int l = t / (K / 4 + 1), m = t - l * (K / 4 + 1);
+      cpx<T> *ln = buf + l * LP;
+      if (m < K / 4) {
+        const int k0 = m, k1 = K / 2 - 1 - m;
+        cpx<T> A0 = ln[k0], B0 = ln[Kh + k0], A1 = ln[k1], B1 = ln[Kh + k1];
+        if (k0 == 0) {
+          A0.y = T(0);
+          B0.y = T(0);
+        }
+        ln[k0] = {A0.x - B0.y, A0.y + B0.x};
+        if (k0 > 0) ln[K - k0] = {A0.x + B0.y, B0.x - A0.y};
+        ln[k1] = {A1.x - B1.y, A1.y + B1.x};
+        ln[K - k1] = {A1.x + B1.y, B1.x - A1.y};
+      } else {
+        xy_pack<T, K>(ln, K / 2);
+      }
+    }
+    __syncthreads();
+    // x inverse, step A in place
+    for (int t = tid; t < L2 * N2; t += NT) {
+      const int l = t / N2, n2 = t - l * N2;
+      cpx<T> *ln = buf + l * LP;
+      cpx<T> a[N1];
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) a[n1] = ln[n1 * N2 + n2];
+      reg_fft<N1, +1>(a);
+#pragma unroll
+      for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<+1>(s_tw, n2 * k1));
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) ln[k1 * N2 + n2] = a[k1];
+    }
+    __syncthreads();
+    // x inverse, step B straight to the real rows 2l (real part), 2l+1 (imag)
+    {
+      const int comp = q / K, z = q - comp * K;
+      T *base = grid + comp * comp_stride + (size_t)z * K * RS;
+      for (int t = tid; t < L2 * N1; t += NT) {
+        const int l = t / N1, k1 = t - l * N1;
+        cpx<T> bb[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) bb[n2] = buf[l * LP + k1 * N2 + n2];
+        reg_fft<N2, +1>(bb);
+        T *ra = base + (size_t)(2 * l) * RS;
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) {
+          ra[k1 + N1 * k2] = bb[k2].x;
+          ra[RS + k1 + N1 * k2] = bb[k2].y;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Influence table (old-version/spme.c:128-150), FP64:
 //   tab[(z*K + y)*Khp + x] = psi(|k|) |b_x|^2 |b_y|^2 |b_z|^2 / V, x <= K/2,
 // psi(k) = 6 pi e^{-k^2/4xi^2} (1 - k^2/3)(1 + k^2/4xi^2 + k^4/8xi^4)/k^2
@@ -336,8 +659,22 @@ int launch_interp(const InterpArgs<T> &a, hipStream_t st) {
   return -1;
 }
 
+// Fused x + y passes for meshes of K >= 128 whose planes fit in LDS (K 128
+// and 192 in FP32, 128 in FP64); otherwise separate x and y passes.  Smaller
+// meshes stay in the separate passes: their spectrum is cache-resident, and
+// the grouped kernels (which use the separate passes) then give the same
+// bits as single-system applies.
+template <typename T, int K> constexpr bool xy_fits() {
+  return K >= 128 && XYShape<T, K>::LDS <= 160 * 1024;
+}
+// Threads per plane block: enough registers per lane (168 at 768 threads)
+// for the next plane's prefetch beside the transform's register tiles.
+template <typename T, int K> constexpr int xy_threads() {
+  return K >= 192 ? 768 : (K >= 128 ? (sizeof(T) == 8 ? 768 : 512) : 256);
+}
+
 template <typename T, int K>
-static void fft3d_k(const FftArgs<T> &a, hipStream_t st, hipEvent_t *ev) {
+static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
   using S = FftShape<T>;
   constexpr int NT = 256;
   const long rows = (long)a.ncomp * K * K;
@@ -345,38 +682,57 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, hipEvent_t *ev) {
   const int Khp = a.RS / 2;
   const int nxc = (K / 2 + 1 + S::XW - 1) / S::XW;
   const int nxcz = (K / 2 + 1 + S::XWZ - 1) / S::XWZ;
-  auto mark = [&](int i) {
-    if (ev) (void)hipEventRecord(ev[i], st);
+  auto mark = [&](int pass) {
+    if (mk) {
+      (void)hipEventRecord(mk->ev[mk->n], st);
+      mk->pass[mk->n++] = pass;
+    }
   };
-  hipLaunchKernelGGL((k_fft_x_r2c<T, K, S::NLX, NT>), dim3(grid1(npairs, S::NLX)), dim3(NT), 0,
-                     st, a.grid, a.RS, npairs, a.tw);
-  mark(0);
-  hipLaunchKernelGGL((k_fft_y<T, K, S::XW, -1, NT>), dim3(nxc * K * a.ncomp), dim3(NT), 0, st,
-                     a.grid, a.comp_stride, Khp, nxc, a.tw);
-  mark(1);
-  hipLaunchKernelGGL((k_fft_z_infl<T, K, S::XWZ, NT>), dim3(nxcz * K * (a.ncomp / 3)), dim3(NT),
-                     0, st, a.grid, a.comp_stride, Khp, nxcz, a.tab, a.dk, a.tw);
-  mark(2);
-  hipLaunchKernelGGL((k_fft_y<T, K, S::XW, +1, NT>), dim3(nxc * K * a.ncomp), dim3(NT), 0, st,
-                     a.grid, a.comp_stride, Khp, nxc, a.tw);
-  mark(3);
-  hipLaunchKernelGGL((k_fft_x_c2r<T, K, S::NLX, NT>), dim3(grid1(npairs, S::NLX)), dim3(NT), 0,
-                     st, a.grid, a.RS, npairs, a.tw);
-  mark(4);
+  if constexpr (xy_fits<T, K>()) {
+    constexpr int NTXY = xy_threads<T, K>();
+    const int nplanes = a.ncomp * K;
+    const int per_cu = std::max(1, std::min(2, int((160 * 1024) / XYShape<T, K>::LDS)));
+    const int grid = std::min(nplanes, std::max(1, a.ncu) * per_cu);
+    hipLaunchKernelGGL((k_fft_xy_fwd<T, K, NTXY>), dim3(grid), dim3(NTXY), 0, st, a.grid, a.RS,
+                       a.comp_stride, nplanes, a.tw);
+    mark(0);
+    hipLaunchKernelGGL((k_fft_z_infl<T, K, S::XWZ, NT>), dim3(nxcz * K * (a.ncomp / 3)),
+                       dim3(NT), 0, st, a.grid, a.comp_stride, Khp, nxcz, a.tab, a.dk, a.tw);
+    mark(2);
+    hipLaunchKernelGGL((k_fft_xy_inv<T, K, NTXY>), dim3(grid), dim3(NTXY), 0, st, a.grid, a.RS,
+                       a.comp_stride, nplanes, a.tw);
+    mark(4);
+  } else {
+    hipLaunchKernelGGL((k_fft_x_r2c<T, K, S::NLX, NT>), dim3(grid1(npairs, S::NLX)), dim3(NT),
+                       0, st, a.grid, a.RS, npairs, a.tw);
+    mark(0);
+    hipLaunchKernelGGL((k_fft_y<T, K, S::XW, -1, NT>), dim3(nxc * K * a.ncomp), dim3(NT), 0,
+                       st, a.grid, a.comp_stride, Khp, nxc, a.tw);
+    mark(1);
+    hipLaunchKernelGGL((k_fft_z_infl<T, K, S::XWZ, NT>), dim3(nxcz * K * (a.ncomp / 3)),
+                       dim3(NT), 0, st, a.grid, a.comp_stride, Khp, nxcz, a.tab, a.dk, a.tw);
+    mark(2);
+    hipLaunchKernelGGL((k_fft_y<T, K, S::XW, +1, NT>), dim3(nxc * K * a.ncomp), dim3(NT), 0,
+                       st, a.grid, a.comp_stride, Khp, nxc, a.tw);
+    mark(3);
+    hipLaunchKernelGGL((k_fft_x_c2r<T, K, S::NLX, NT>), dim3(grid1(npairs, S::NLX)), dim3(NT),
+                       0, st, a.grid, a.RS, npairs, a.tw);
+    mark(4);
+  }
 }
 
 template <typename T>
-int launch_fft_influence(const FftArgs<T> &a, hipStream_t st, hipEvent_t *ev) {
+int launch_fft_influence(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
   switch (a.K) {
-    case 16: fft3d_k<T, 16>(a, st, ev); return 0;
-    case 24: fft3d_k<T, 24>(a, st, ev); return 0;
-    case 32: fft3d_k<T, 32>(a, st, ev); return 0;
-    case 48: fft3d_k<T, 48>(a, st, ev); return 0;
-    case 64: fft3d_k<T, 64>(a, st, ev); return 0;
-    case 96: fft3d_k<T, 96>(a, st, ev); return 0;
-    case 128: fft3d_k<T, 128>(a, st, ev); return 0;
-    case 192: fft3d_k<T, 192>(a, st, ev); return 0;
-    case 256: fft3d_k<T, 256>(a, st, ev); return 0;
+    case 16: fft3d_k<T, 16>(a, st, mk); return 0;
+    case 24: fft3d_k<T, 24>(a, st, mk); return 0;
+    case 32: fft3d_k<T, 32>(a, st, mk); return 0;
+    case 48: fft3d_k<T, 48>(a, st, mk); return 0;
+    case 64: fft3d_k<T, 64>(a, st, mk); return 0;
+    case 96: fft3d_k<T, 96>(a, st, mk); return 0;
+    case 128: fft3d_k<T, 128>(a, st, mk); return 0;
+    case 192: fft3d_k<T, 192>(a, st, mk); return 0;
+    case 256: fft3d_k<T, 256>(a, st, mk); return 0;
   }
   return -1;
 }
@@ -398,7 +754,7 @@ template int launch_spread<float>(const SpreadArgs<float> &, hipStream_t);
 template int launch_spread<double>(const SpreadArgs<double> &, hipStream_t);
 template int launch_interp<float>(const InterpArgs<float> &, hipStream_t);
 template int launch_interp<double>(const InterpArgs<double> &, hipStream_t);
-template int launch_fft_influence<float>(const FftArgs<float> &, hipStream_t, hipEvent_t *);
-template int launch_fft_influence<double>(const FftArgs<double> &, hipStream_t, hipEvent_t *);
+template int launch_fft_influence<float>(const FftArgs<float> &, hipStream_t, FftMarks *);
+template int launch_fft_influence<double>(const FftArgs<double> &, hipStream_t, FftMarks *);
 
 }  // namespace spme

@@ -64,6 +64,16 @@ template <typename T> struct FftArgs {
   const T *tab;
   T dk;
   const cpx<T> *tw;
+  int ncu;  // compute units (grid of the fused plane passes)
+};
+
+// Events recorded after each FFT launch and the pass each one ends
+// (0 x or fused x+y forward, 1 y forward, 2 z + influence, 3 y inverse,
+// 4 x or fused x+y inverse).
+struct FftMarks {
+  hipEvent_t ev[5];
+  int pass[5];
+  int n;
 };
 
 void launch_pbuild(const double *pos, int N, double L, int K, int P, const WTable &W, int TX,
@@ -81,9 +91,9 @@ void launch_permute_in(const int *perm, const T *in, long ldin, T *fs, int N, in
                        hipStream_t st);
 template <typename T> int launch_spread(const SpreadArgs<T> &a, hipStream_t st);
 template <typename T> int launch_interp(const InterpArgs<T> &a, hipStream_t st);
-// ev (optional): 5 events recorded after each of the 5 FFT-pass launches.
+// mk (optional): events to record after each launch (mk->n = 0 on entry).
 template <typename T>
-int launch_fft_influence(const FftArgs<T> &a, hipStream_t st, hipEvent_t *ev = nullptr);
+int launch_fft_influence(const FftArgs<T> &a, hipStream_t st, FftMarks *mk = nullptr);
 bool fft_size_supported(int K);
 template <typename T>
 void launch_influence_table(const double *bsq, int K, int Khp, double dk, double xi,

@@ -89,6 +89,35 @@ def test_spread_and_fft_stages(oracle, E, torch, p, K):
     eng.close()
 
 
+@pytest.mark.parametrize("prec,K", [("fp64", 96), ("fp64", 128), ("fp64", 192),
+                                    ("fp32", 128), ("fp32", 192), ("fp32", 256)])
+def test_fft_pipeline_large_meshes(oracle, E, torch, prec, K):
+    """Forward transform, influence and inverse transform on the engine's own
+    spread mesh, at the sizes where whole planes are transformed in LDS (FP64
+    K <= 128, FP32 K <= 192) and beyond (separate x and y passes).  The
+    reference transforms are numpy's rfftn / irfftn (the c2r drops the
+    imaginary parts of the kx = 0 and K/2 terms, as the engine's does) around
+    the oracle's influence step."""
+    _say(f"case: fft pipeline {prec} K={K} N=2000 p=6")
+    N, p = 2000, 6
+    pos, L = _system(300 + K, N, 0.2)
+    xi = 0.45
+    f = W.forces(300 + K, N)[0]
+    P = E.SPME_FP64 if prec == "fp64" else E.SPME_FP32
+    dt = torch.float64 if prec == "fp64" else torch.float32
+    eng = E.Engine(xi, K, p, N, L, precision=P)
+    eng.set_positions(pos)
+    ft = torch.tensor(f, device="cuda", dtype=dt).view(1, -1)
+    m1 = eng.debug_stage(1, ft)[:, :, :, :K].astype(np.float64)
+    m2 = eng.debug_stage(2, ft)[:, :, :, :K]
+    eng.close()
+    lm2 = oracle.compute_lm2(xi, K, p, L)
+    spec = np.stack([np.fft.rfftn(m1[c]) for c in range(3)])
+    spec = oracle.apply_influence(K, L, lm2, spec)
+    back = np.stack([np.fft.irfftn(spec[c], s=(K, K, K), axes=(0, 1, 2)) for c in range(3)]) * (K ** 3 / L ** 3)
+    assert _rel(m2, back) < (FP64_TOL if prec == "fp64" else FP32_TOL)
+
+
 @pytest.mark.parametrize("K,p", [(32, 4), (48, 6), (64, 8)])
 def test_recip_fp64_and_fp32(oracle, E, torch, K, p):
     _say(f"case: recip K={K} p={p} N=1500 fp64+fp32")

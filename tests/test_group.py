@@ -1,7 +1,8 @@
 """Grouped multi-system applies, tile shapes and the autotuner.
 
-A group runs the same device bodies as the separate engines, and no tile
-shape changes the arithmetic, so every comparison here is bitwise:
+A group runs the same device bodies as the separate engines (for meshes up
+to K = 96; larger single applies fuse the x and y FFT passes), and no tile
+shape changes the arithmetic, so the comparisons are bitwise:
   - grouped apply (plain launches and one captured hipGraph, replayed with
     eager work and synchronisation between replays) == the systems'
     separate spme_engine_apply results, for the reciprocal part and the
@@ -68,6 +69,19 @@ def _separate(engines, fs, parts, alpha=1.0, beta=0.0, u0=None):
 def _equal(a, b):
     torch = _torch()
     return all(torch.equal(x, y) for x, y in zip(a, b))
+
+
+def _close(a, b, engines, tol=2e-5):
+    """Bitwise for members with K <= 96; within tol (relative max norm) for
+    larger ones, whose single applies fuse the x and y FFT passes."""
+    torch = _torch()
+    for x, y, e in zip(a, b, engines):
+        if e.info().K <= 96:
+            if not torch.equal(x, y):
+                return False
+        elif float((x - y).abs().max() / y.abs().max()) > tol:
+            return False
+    return True
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
@@ -168,14 +182,14 @@ def test_group_follows_new_positions_and_large_meshes():
     us = [torch.zeros_like(f) for f in fs]
     g.apply(fs, us, parts=E.SPME_FULL, graph=True)
     torch.cuda.synchronize()
-    assert _equal(us, _separate(engines, fs, E.SPME_FULL))
+    assert _close(us, _separate(engines, fs, E.SPME_FULL), engines)
     m = members[2]
     engines[2].set_positions(W.positions(m.seed + 1, m.n, m.L))
     ref = _separate(engines, fs, E.SPME_FULL)
     us = [torch.zeros_like(f) for f in fs]
     g.apply(fs, us, parts=E.SPME_FULL, graph=True)
     torch.cuda.synchronize()
-    assert _equal(us, ref)
+    assert _close(us, ref, engines)
     g.close()
 
 
