@@ -256,7 +256,7 @@ __global__ __launch_bounds__(NT) void k_fft_xy_fwd(T *__restrict__ grid, int RS,
   __shared__ cpx<T> s_tw[K];
   using V = Vec16<T>;
   load_tw<T, K>(s_tw, tw);
-  const int tid = threadIdx.x;
+  int tid = threadIdx.x;
   const int Khp = RS / 2;
   V ra[NU], rb[NU];
   auto fetch = [&](int q) {
@@ -286,8 +286,13 @@ __global__ __launch_bounds__(NT) void k_fft_xy_fwd(T *__restrict__ grid, int RS,
   int q = blockIdx.x;
   if (q < nplanes) fetch(q);
   for (; q < nplanes; q += gridDim.x) {
+    // keep per-thread index arithmetic inside the loop: hoisted, it would
+    // stay live across the transform and spill
+    asm volatile("" : "+v"(tid));
     commit();
     __syncthreads();
+    // the next plane's loads, in flight through this one
+    if (q + (int)gridDim.x < nplanes) fetch(q + gridDim.x);
     // x, step A: per line and n2, an N1-point FFT, twiddled, in place
     for (int t = tid; t < L2 * N2; t += NT) {
       const int l = t / N2, n2 = t - l * N2;
@@ -302,32 +307,25 @@ __global__ __launch_bounds__(NT) void k_fft_xy_fwd(T *__restrict__ grid, int RS,
       for (int k1 = 0; k1 < N1; ++k1) ln[k1 * N2 + n2] = a[k1];
     }
     __syncthreads();
-    // x, step B: N2-point FFTs into registers, then natural order
-    constexpr int PB = (L2 * N1 + NT - 1) / NT;
-    cpx<T> b[PB][N2];
+    // x, step B, natural order: in rounds of whole lines (a line's tasks
+    // read and write the same positions), one register tile per thread
+    constexpr int LPR = NT / N1;  // lines per round
+    for (int l0 = 0; l0 < L2; l0 += LPR) {
+      const int l = l0 + tid / N1, k1 = tid % N1;
+      const bool on = tid < LPR * N1 && l < L2;
+      cpx<T> b[N2];
+      if (on) {
 #pragma unroll
-    for (int u = 0; u < PB; ++u) {
-      const int t = tid + u * NT;
-      if (t < L2 * N1) {
-        const int l = t / N1, k1 = t - l * N1;
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) b[u][n2] = buf[l * LP + k1 * N2 + n2];
-        reg_fft<N2, -1>(b[u]);
+        for (int n2 = 0; n2 < N2; ++n2) b[n2] = buf[l * LP + k1 * N2 + n2];
+        reg_fft<N2, -1>(b);
       }
-    }
-    __syncthreads();
+      __syncthreads();
+      if (on) {
 #pragma unroll
-    for (int u = 0; u < PB; ++u) {
-      const int t = tid + u * NT;
-      if (t < L2 * N1) {
-        const int l = t / N1, k1 = t - l * N1;
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) buf[l * LP + k1 + N1 * k2] = b[u][k2];
+        for (int k2 = 0; k2 < N2; ++k2) buf[l * LP + k1 + N1 * k2] = b[k2];
       }
+      __syncthreads();
     }
-    // the next plane's loads, in flight through the rest of this one
-    if (q + (int)gridDim.x < nplanes) fetch(q + gridDim.x);
-    __syncthreads();
     // split into half spectra: k and K/2-1-k per thread, K/2 alone
     for (int t = tid; t < L2 * (K / 4 + 1); t += NT) {
       const int l = t / (K / 4 + 1), m = t - l * (K / 4 + 1);
@@ -390,7 +388,7 @@ __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS,
   __shared__ cpx<T> s_tw[K];
   struct alignas(16) C16 { cpx<T> v[G]; };
   load_tw<T, K>(s_tw, tw);
-  const int tid = threadIdx.x;
+  int tid = threadIdx.x;
   const int Khp = RS / 2;
   C16 rv[NU];
   auto fetch = [&](int q) {
@@ -421,8 +419,13 @@ __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS,
   int q = blockIdx.x;
   if (q < nplanes) fetch(q);
   for (; q < nplanes; q += gridDim.x) {
+    // keep per-thread index arithmetic inside the loop: hoisted, it would
+    // stay live across the transform and spill
+    asm volatile("" : "+v"(tid));
     commit();
     __syncthreads();
+    // the next plane's loads, in flight through this one
+    if (q + (int)gridDim.x < nplanes) fetch(q + gridDim.x);
     // y inverse, step A in place
     for (int t = tid; t < Kh * N2; t += NT) {
       const int x = t % Kh, n2 = t / Kh;
@@ -436,32 +439,26 @@ __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS,
       for (int k1 = 0; k1 < N1; ++k1) buf[(k1 * N2 + n2) * Kh + x] = a[k1];
     }
     __syncthreads();
-    // y inverse, step B into registers, then natural order
-    constexpr int PY = (Kh * N1 + NT - 1) / NT;
-    cpx<T> c[PY][N2];
+    // y inverse, step B, natural order: in rounds of whole columns (a
+    // column's tasks read and write the same positions), one register tile
+    // per thread
+    constexpr int CPR = NT / N1;  // columns per round
+    for (int x0 = 0; x0 < Kh; x0 += CPR) {
+      const int x = x0 + tid / N1, k1 = tid % N1;
+      const bool on = tid < CPR * N1 && x < Kh;
+      cpx<T> c[N2];
+      if (on) {
 #pragma unroll
-    for (int u = 0; u < PY; ++u) {
-      const int t = tid + u * NT;
-      if (t < Kh * N1) {
-        const int x = t % Kh, k1 = t / Kh;
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) c[u][n2] = buf[(k1 * N2 + n2) * Kh + x];
-        reg_fft<N2, +1>(c[u]);
+        for (int n2 = 0; n2 < N2; ++n2) c[n2] = buf[(k1 * N2 + n2) * Kh + x];
+        reg_fft<N2, +1>(c);
       }
-    }
-    __syncthreads();
+      __syncthreads();
+      if (on) {
 #pragma unroll
-    for (int u = 0; u < PY; ++u) {
-      const int t = tid + u * NT;
-      if (t < Kh * N1) {
-        const int x = t % Kh, k1 = t / Kh;
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) buf[(k1 + N1 * k2) * Kh + x] = c[u][k2];
+        for (int k2 = 0; k2 < N2; ++k2) buf[(k1 + N1 * k2) * Kh + x] = c[k2];
       }
+      __syncthreads();
     }
-    // the next plane's loads, in flight through the rest of this one
-    if (q + (int)gridDim.x < nplanes) fetch(q + gridDim.x);
-    __syncthreads();
     // pack row pairs into lines: k and K/2-1-k per thread, K/2 alone
     for (int t = tid; t < L2 * (K / 4 + 1); t += NT) {
       const int l = t / (K / 4 + 1), m = t - l * (K / 4 + 1);
