@@ -9,14 +9,20 @@ space + overlap + self) to 16 force vectors = 16 M.f.  Inputs are resident
 in HBM before the timed region.  Per-kernel durations come from HIP events
 the engine records on the apply stream during the timed region.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+A second leg times the uneven multi-system workload (64 boxes, N in
+[2e3, 5e4], K in {32, 48, 64, 96}, p = 6, 4 right-hand sides, FP32): 64
+separate applies against one grouped launch per stage and against one
+captured hipGraph.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-multi]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
-import math
 import os
+import re
 import sys
 import time
 
@@ -26,6 +32,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+TOL = 1e-4             # target relative error of the batched form
+MARGIN = 1.5           # the selection is asked for TOL / MARGIN
+# rocprof kernel name of each timed stage (FP32, K = 192, p = 8, 16 RHS)
+ROCPROF_NAME = {
+    "permute": "k_permute_in<float>",
+    "spread": "k_spread<float, 8, 12, 8, 8>",
+    "fft_x_fwd": "k_fft_xy_fwd<float, 192, 768>",
+    "fft_z_infl": "k_fft_z_infl<float, 192, 8, 256>",
+    "fft_x_inv": "k_fft_xy_inv<float, 192, 768>",
+    "interp": "k_interp<float, 8, 4, 16, 1024>",
+    "permute_real": "k_permute_in<float>",
+    "real": "k_real_spmv<float, 48>",
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c3.md")
 
 
 def parse():
@@ -36,6 +56,8 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline leg")
+    ap.add_argument("--no-multi", action="store_true",
+                    help="skip the uneven multi-system leg")
     ap.add_argument("--cpu-rhs", type=int, default=1)
     ap.add_argument("--pmc-run", action="store_true",
                     help="short run for hardware-counter passes: a plain copy of "
@@ -43,9 +65,25 @@ def parse():
     return ap.parse_args()
 
 
+def c3_params():
+    """(xi, rc, note) for the batched workload, whose mesh and order are
+    fixed (K = 192, p = 8).  The learned selector picks K and p itself and
+    was trained on N <= 1e5, so the analytic error estimate sets xi and rc,
+    asked for TOL / MARGIN."""
+    from spme_amd import params as PR
+    from spme_amd import workload as W
+    cfg = W.CONFIGS["c3"]
+    a = PR.analytic_select(cfg["n"], cfg["phi"], TOL / MARGIN, nrhs=cfg["nrhs"],
+                           fixed_K=cfg["K"], fixed_p=cfg["p"])
+    note = (f"analytic estimate for rel. error {TOL:g}/{MARGIN:g} at fixed K={cfg['K']}, "
+            f"p={cfg['p']} (learned selector: N=200000 is outside its training range)")
+    return a.xi, a.rc, note
+
+
 def pmc_run():
-    """Workload for rocprofv3 --pmc passes.  The copy moves exactly 2^28
-    bytes each way so FETCH_SIZE/WRITE_SIZE can be scaled per byte."""
+    """Workload for rocprofv3 --pmc passes.  The fill writes and the copy
+    moves exactly 2^28 bytes each way, so FETCH_SIZE/WRITE_SIZE can be
+    scaled per byte."""
     import torch
     from spme_amd import engine as E
     from spme_amd import workload as W
@@ -60,7 +98,8 @@ def pmc_run():
     s = W.make_system(cfg["seed"], cfg["n"], cfg["phi"])
     f = torch.tensor(W.forces(cfg["seed"], cfg["n"], cfg["nrhs"]), device="cuda:0",
                      dtype=torch.float32)
-    eng = E.Engine(0.46, cfg["K"], cfg["p"], cfg["n"], s.L, rc=8.0,
+    xi, rc, _ = c3_params()
+    eng = E.Engine(xi, cfg["K"], cfg["p"], cfg["n"], s.L, rc=rc,
                    precision=E.SPME_FP32, max_rhs=cfg["nrhs"])
     eng.set_positions(torch.tensor(s.pos, device="cuda:0"))
     u = torch.zeros_like(f)
@@ -71,10 +110,11 @@ def pmc_run():
     print("[pmc] done", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(N, K, p, R, s):
-    """Compulsory HBM bytes per launch of each kernel (reciprocal part as in
-    the byte model of the path: b_P = 3p*s + 12 per particle, Kh = (K/2+1)K^2,
-    C = 3R components)."""
+def algorithmic_bytes(N, K, p, R, s, real_entries):
+    """Compulsory HBM bytes per launch of each kernel: b_P = 3p*s + 12 per
+    particle, Kh = (K/2+1)K^2, C = 3R components; the real-space SpMV reads
+    each stored pair entry once (6 block values + a column index) and each
+    force and output vector once."""
     C = 3 * R
     K3 = K ** 3
     Kh = (K // 2 + 1) * K * K
@@ -83,16 +123,92 @@ def algorithmic_bytes(N, K, p, R, s):
     return {
         "permute": 2 * vec + 4 * N,
         "spread": vec + N * bP + K3 * C * s,
+        # fused x + y passes: real mesh in, half spectrum out (and back)
         "fft_x_fwd": K3 * C * s + Kh * C * 2 * s,
-        "fft_y_fwd": 2 * Kh * C * 2 * s,
         "fft_z_infl": 2 * Kh * C * 2 * s + Kh * s,
-        "fft_y_inv": 2 * Kh * C * 2 * s,
         "fft_x_inv": Kh * C * 2 * s + K3 * C * s,
         "interp": K3 * C * s + N * bP + vec,
         "permute_real": 2 * vec + 4 * N,
-        "real": 24 * N + 3 * vec,
+        "real": real_entries * (6 * s + 4) + 3 * vec + 4 * N,
         # the whole forward + inverse transform as one unit
         "fft_total": 2 * (K3 * C * s + Kh * C * 2 * s),
+    }
+
+
+def traffic_per_launch(kernel):
+    """HBM bytes per launch of a kernel from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        m = re.match(r"\|\s*`([^`]+)`\s*\|.*\|\s*([0-9.]+)\s*\|\s*$", line)
+        if m and m.group(1) == kernel:
+            return float(m.group(2)) * 1e6  # the table gives MB
+    return None
+
+
+def multi_system(steps, warmup):
+    """64 uneven systems: separate applies vs grouped launches vs one graph."""
+    import torch
+    from spme_amd import engine as E
+    from spme_amd import workload as W
+    members = W.multi_systems()
+    R = W.CONFIGS["c4"]["nrhs"]
+    engines, fs, us = [], [], []
+    for m in members:
+        e = E.Engine(m.xi, m.K, m.p, m.n, m.L, rc=m.rc, precision=E.SPME_FP32, max_rhs=R)
+        e.set_positions(W.positions(m.seed, m.n, m.L))
+        engines.append(e)
+        fs.append(torch.tensor(W.forces(m.seed, m.n, R), device="cuda:0", dtype=torch.float32))
+        us.append(torch.zeros_like(fs[-1]))
+    lib = E.load()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    calls = [(e._h, ctypes.c_void_p(f.data_ptr()), ctypes.c_long(f.stride(0)),
+              ctypes.c_void_p(u.data_ptr()), ctypes.c_long(u.stride(0)))
+             for e, f, u in zip(engines, fs, us)]
+    apply_fn = lib.spme_engine_apply
+
+    def separate():
+        for h, fi, li, uo, lo in calls:
+            apply_fn(h, R, 1.0, fi, li, 0.0, uo, lo, E.SPME_FULL, st)
+
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(steps):
+            fn()
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / steps
+
+    ms_sep = timed(separate)
+    ref = [u.clone() for u in us]
+    g = E.Group(engines, R)
+    tiles = g.autotune(parts=E.SPME_FULL, reps=3)
+    ms_grp = timed(lambda: g.apply(fs, us, parts=E.SPME_FULL, graph=False))
+    ms_graph = timed(lambda: g.apply(fs, us, parts=E.SPME_FULL, graph=True))
+    bitwise = all(torch.equal(a, b) for a, b in zip(us, ref))
+    g.close()
+    for e in engines:
+        e.close()
+    mf = len(members) * R
+    return {
+        "workload": f"{len(members)} uneven periodic boxes, N in [2000, 50000] "
+                    f"(total {sum(m.n for m in members)}), phi in [0.05, 0.3], "
+                    f"K in {{32, 48, 64, 96}}, p=6, {R} RHS each, FP32, full operator",
+        "meshes": {str(k): sum(1 for m in members if m.K == k) for k in (32, 48, 64, 96)},
+        "separate_ms": round(ms_sep, 4),
+        "grouped_ms": round(ms_grp, 4),
+        "graph_ms": round(ms_graph, 4),
+        "separate_Mf_per_s": round(mf / ms_sep * 1e3, 1),
+        "graph_Mf_per_s": round(mf / ms_graph * 1e3, 1),
+        "speedup_graph_vs_separate": round(ms_sep / ms_graph, 3),
+        "bitwise_equal_to_separate": bitwise,
+        "tiles": tiles,
     }
 
 
@@ -103,8 +219,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU "
-                          "sharding is a design note only (see the design "
-                          "document), so --gpus must be 1"}))
+                          "sharding is a design note only, so --gpus must be 1"}))
         sys.exit(2)
     import torch
     from spme_amd import engine as E
@@ -114,8 +229,7 @@ def main():
     N, K, p, R = cfg["n"], cfg["K"], cfg["p"], cfg["nrhs"]
     sysm = W.make_system(cfg["seed"], N, cfg["phi"])
     L = sysm.L
-    # parameters for a 1e-4 target with the mesh and order fixed at K, p
-    xi, rc = 0.46, 8.0
+    xi, rc, params_note = c3_params()
     dev = "cuda:0"
     torch.cuda.set_device(0)
     f64 = W.forces(cfg["seed"], N, R).astype(np.float32).astype(np.float64)
@@ -130,10 +244,13 @@ def main():
     eng.set_positions(pos_d)
     torch.cuda.synchronize()
     t_pbuild = time.perf_counter() - t0
+    tiles = eng.autotune(nrhs=R, reps=3)
+    real_entries = eng.info().real_entries
     for _ in range(args.warmup):
         eng.apply(f32_d, u32_d, parts=E.SPME_FULL)
     torch.cuda.synchronize()
-    print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
+    print(f"[bench] warmup done ({args.warmup} steps), tiles {tiles}", file=sys.stderr,
+          flush=True)
     eng.set_timing(True)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -151,10 +268,6 @@ def main():
     value = R * args.steps / (ms_total / 1e3)
     print(f"[bench] fp32 batched: {ms_step:.3f} ms/step, {value:.1f} M.f/s "
           f"(wall {wall:.3f}s)", file=sys.stderr, flush=True)
-
-    # recip-only timing for the breakdown (not the headline)
-    u_rec = eng.apply(f32_d, torch.zeros_like(f32_d), parts=E.SPME_RECIP)
-    torch.cuda.synchronize()
     u_full = u32_d.double().cpu().numpy()
     eng.close()
 
@@ -166,7 +279,6 @@ def main():
     u64_d = torch.zeros_like(f64_d)
     for _ in range(max(1, args.warmup)):
         engp.apply(f64_d[0], u64_d[0], parts=E.SPME_FULL)
-    engp.set_timing(True)
     torch.cuda.synchronize()
     ev0.record(stream)
     for k in range(plain_steps):
@@ -175,7 +287,6 @@ def main():
     ev1.record(stream)
     torch.cuda.synchronize()
     ms_plain = ev0.elapsed_time(ev1) / plain_steps
-    stages_plain = engp.timing()
     for r in range(2):
         engp.apply(f64_d[r], u64_d[r], parts=E.SPME_FULL)
     torch.cuda.synchronize()
@@ -200,7 +311,7 @@ def main():
                   "rows": int(rows.size), "rhs_checked": int(direct.shape[0])}
 
     # ---- roofline of the dominant kernel -------------------------------------
-    ab = algorithmic_bytes(N, K, p, R, 4)
+    ab = algorithmic_bytes(N, K, p, R, 4, real_entries)
     per = {}
     for name, (ms, n) in stages.items():
         if n:
@@ -209,8 +320,18 @@ def main():
                          "GBs": round(ab[name] / (avg * 1e-3) / 1e9, 1)}
     dom = max(per, key=lambda k: per[k]["avg_ms"])
     dom_gbs = ab[dom] / (per[dom]["avg_ms"] * 1e-3) / 1e9
+    traffic = traffic_per_launch(ROCPROF_NAME.get(dom, ""))
     fft_ms = sum(per[k]["avg_ms"] for k in per if k.startswith("fft_"))
     fft_gbs = ab["fft_total"] / (fft_ms * 1e-3) / 1e9
+    mesh_ms = fft_ms + per["spread"]["avg_ms"] + per["interp"]["avg_ms"]
+    mesh_bytes = ab["fft_total"] + ab["spread"] + ab["interp"]
+
+    # ---- uneven multi-system leg -------------------------------------------
+    multi = None
+    if not args.no_multi:
+        multi = multi_system(max(3, args.steps // 4), 2)
+        print(f"[bench] multi-system: separate {multi['separate_ms']:.3f} ms, graph "
+              f"{multi['graph_ms']:.3f} ms", file=sys.stderr, flush=True)
 
     # ---- CPU baseline: restated reference on the host cores ----------------
     cpu = None
@@ -223,7 +344,7 @@ def main():
         cpu = {"value": round(nr / tc, 4), "unit": "M.f/s", "cores": O.num_threads(),
                "kind": "port",
                "sample": f"{nr} right-hand side(s) of the same N=200k, K=192, p=8, "
-                         f"rc={rc} FP64 full M.f by the restated reference algorithm "
+                         f"rc={rc:.3f} FP64 full M.f by the restated reference algorithm "
                          f"(P build included), {tc:.1f} s"}
         print(f"[bench] cpu: {tc:.2f}s for {nr} M.f on {O.num_threads()} threads",
               file=sys.stderr, flush=True)
@@ -243,19 +364,27 @@ def main():
         "data": "synthetic (seeded uniform positions, N(0,1) forces)",
         "config": {"workload": "RPY M.f, N=200000 unit spheres, phi=0.2, "
                                "L=161.199, K=192, p=8, 16 RHS batched, full operator",
-                   "xi": xi, "rc": rc, "nrhs": R},
+                   "xi": round(xi, 6), "rc": round(rc, 4), "nrhs": R,
+                   "params": params_note},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(dom_gbs, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": None if traffic is None else round(traffic)},
+        "mesh_roofline": {"kernels": "spread + FFT passes + interpolation",
+                          "achieved": round(mesh_bytes / (mesh_ms * 1e-3) / 1e9, 1),
+                          "frac": round(mesh_bytes / (mesh_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                          "ms": round(mesh_ms, 4)},
         "fft_roofline": {"achieved": round(fft_gbs, 1),
                          "frac": round(fft_gbs / HBM_PEAK_GBS, 4),
                          "ms": round(fft_ms, 4)},
         "kernels": per,
+        "tiles": tiles,
         "plain_fp64": {"value": round(plain_value, 2), "unit": "M.f/s",
                        "ms_per_mf": round(ms_plain, 4),
                        "speedup_of_batched_fp32": round(value / plain_value, 2)},
         "p_build_ms": round(t_pbuild * 1e3, 2),
         "rel_error_vs_fp64_direct_ewald": errors,
+        "multi_system": multi,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -160,6 +160,8 @@ typedef struct spme_info {
   double L, xi, rc;
   long mesh_bytes;     /* device bytes of the mesh */
   void *stream;        /* the engine's own stream */
+  long real_entries;   /* stored real-space pair entries (rows padded to
+                          their 64-row slice's width); 0 before positions */
 } spme_info_t;
 int spme_engine_info(const spme_t *spme, spme_info_t *info);
 

@@ -718,21 +718,25 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
   }
 }
 
-// One row (cell order) of the real-space SpMV.  fs: forces in cell order [N][C].
-// out[r*ldout + 3*perm[s] + d] = (accum ? out : beta*out) + alpha*(self f + sum_j B_sj f_j).
-// Rows are packed in slices of 64 (entry-major within a slice).
+// Real-space SpMV, one lane per (row, right-hand side): the R = C/3 lanes
+// of a row (cell order) read a neighbour's forces as one contiguous segment
+// and share the pair's block and column index.  fs: forces in cell order
+// [N][C].  For right-hand side r and d < 3:
+//   out[r*ldout + 3*perm[s] + d] = (accum ? out : beta*out)
+//                                  + alpha*(self f + sum_j B_sj f_j).
+// Rows are packed in slices of 64 (entry-major within a slice); the rows of
+// a wave lie in one slice, so every lane runs the same entry count.
 template <typename T, int C>
-__device__ __forceinline__ void real_row(const int *__restrict__ perm,
-                                         const int *__restrict__ width,
-                                         const int *__restrict__ soff,
-                                         const int *__restrict__ col,
-                                         const T *__restrict__ blk, long E, T self,
-                                         const T *__restrict__ fs, T alpha, T beta, int accum,
-                                         T *__restrict__ out, long ldout, int s) {
+__device__ __forceinline__ void real_lane(const int *__restrict__ perm,
+                                          const int *__restrict__ width,
+                                          const int *__restrict__ soff,
+                                          const int *__restrict__ col,
+                                          const T *__restrict__ blk, long E, T self,
+                                          const T *__restrict__ fs, T alpha, T beta, int accum,
+                                          T *__restrict__ out, long ldout, int s, int r) {
   const int q = s >> 6, l = s & 63;
-  T acc[C];
-#pragma unroll
-  for (int c = 0; c < C; ++c) acc[c] = self * fs[(size_t)s * C + c];
+  const T *fr = fs + (size_t)s * C + 3 * r;
+  T a0 = self * fr[0], a1 = self * fr[1], a2 = self * fr[2];
   const long base = soff[q] + l;
   const int w = width[q];
 #pragma unroll 4
@@ -741,22 +745,18 @@ __device__ __forceinline__ void real_row(const int *__restrict__ perm,
     const int j = col[e];
     const T b0 = blk[e], b1 = blk[E + e], b2 = blk[2 * E + e];
     const T b3 = blk[3 * E + e], b4 = blk[4 * E + e], b5 = blk[5 * E + e];
-    const T *fj = fs + (size_t)j * C;
-#pragma unroll
-    for (int r = 0; r < C / 3; ++r) {
-      const T f0 = fj[3 * r], f1 = fj[3 * r + 1], f2 = fj[3 * r + 2];
-      acc[3 * r + 0] += b0 * f0 + b1 * f1 + b2 * f2;
-      acc[3 * r + 1] += b1 * f0 + b3 * f1 + b4 * f2;
-      acc[3 * r + 2] += b2 * f0 + b4 * f1 + b5 * f2;
-    }
+    const T *fj = fs + (size_t)j * C + 3 * r;
+    const T f0 = fj[0], f1 = fj[1], f2 = fj[2];
+    a0 += b0 * f0 + b1 * f1 + b2 * f2;
+    a1 += b1 * f0 + b3 * f1 + b4 * f2;
+    a2 += b2 * f0 + b4 * f1 + b5 * f2;
   }
-  const int o = perm[s];
+  T *u = out + r * ldout + 3L * perm[s];
+  const T acc[3] = {a0, a1, a2};
 #pragma unroll
-  for (int c = 0; c < C; ++c) {
-    const int r = c / 3, d = c - 3 * r;
-    T *u = out + r * ldout + 3L * o + d;
-    const T prev = accum ? *u : (beta == T(0) ? T(0) : beta * *u);
-    *u = prev + alpha * acc[c];
+  for (int d = 0; d < 3; ++d) {
+    const T prev = accum ? u[d] : (beta == T(0) ? T(0) : beta * u[d]);
+    u[d] = prev + alpha * acc[d];
   }
 }
 

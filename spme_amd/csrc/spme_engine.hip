@@ -147,7 +147,7 @@ struct _spme_t {
   int nslices;
   int *r_cnt, *r_width, *r_size, *r_soff, *r_col;
   void *r_blk;
-  long r_E, r_cap;
+  long r_E, r_cap, r_used;
   // drop-in staging (FP64 host API)
   double *h_in, *h_out;
   // per-kernel timing: when enabled, events are recorded on the apply
@@ -345,6 +345,7 @@ int build_P(_spme_t *e, hipStream_t st) {
       e->r_cap = cap;
     }
     e->r_E = e->r_cap;
+    e->r_used = total;
     launch_pair_fill<T>(e->r_pos, e->r_off, e->nc, L, e->prm.xi, rc * rc, cut * cut, N,
                         e->r_width, e->r_soff, e->r_col, (T *)e->r_blk, e->r_E, st);
   }
@@ -677,7 +678,7 @@ int group_stage_blocks(const _spme_group_t *g, const _spme_t *e, int stage) {
     case GS_INTERP:
       return interp_blocks(e->K, tl.interp_zc, C,
                            g->tsz == 4 ? InterpShape<float>::CC : InterpShape<double>::CC, g->txy);
-    case GS_REAL: return group_real_blocks(e->N);
+    case GS_REAL: return group_real_blocks(e->N, C);
     default:
       return g->tsz == 4 ? fft_pass_blocks<float>(e->K, C, stage - GS_FFT0)
                          : fft_pass_blocks<double>(e->K, C, stage - GS_FFT0);
@@ -1075,6 +1076,7 @@ int spme_engine_info(const spme_t *e, spme_info_t *info) {
   info->rc = e->prm.rc;
   info->mesh_bytes = (long)(3 * e->max_rhs) * e->comp_stride * (long)e->tsz;
   info->stream = e->stream;
+  info->real_entries = e->have_pos && e->nc >= 3 ? e->r_used : 0;
   return 0;
 }
 

@@ -47,7 +47,7 @@ class SpmeInfo(ctypes.Structure):
                 ("RS", ctypes.c_int), ("comp_stride", ctypes.c_long),
                 ("L", ctypes.c_double), ("xi", ctypes.c_double),
                 ("rc", ctypes.c_double), ("mesh_bytes", ctypes.c_long),
-                ("stream", ctypes.c_void_p)]
+                ("stream", ctypes.c_void_p), ("real_entries", ctypes.c_long)]
 
 
 class SpmeTiles(ctypes.Structure):
