@@ -739,7 +739,7 @@ __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
   T a0 = self * fr[0], a1 = self * fr[1], a2 = self * fr[2];
   const long base = soff[q] + l;
   const int w = width[q];
-#pragma unroll 4
+#pragma unroll 8
   for (int k = 0; k < w; ++k) {
     const long e = base + 64L * k;
     const int j = col[e];

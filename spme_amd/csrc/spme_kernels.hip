@@ -444,7 +444,7 @@ __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS,
     // per thread
     constexpr int CPR = NT / N1;  // columns per round
     for (int x0 = 0; x0 < Kh; x0 += CPR) {
-      const int x = x0 + tid / N1, k1 = tid % N1;
+      const int x = x0 + tid % CPR, k1 = tid / CPR;  // consecutive lanes: consecutive x
       const bool on = tid < CPR * N1 && x < Kh;
       cpx<T> c[N2];
       if (on) {
