@@ -118,6 +118,41 @@ def test_fft_pipeline_large_meshes(oracle, E, torch, prec, K):
     assert _rel(m2, back) < (FP64_TOL if prec == "fp64" else FP32_TOL)
 
 
+def test_c2_fp64_fp32_vs_oracle_and_direct(oracle, E, torch):
+    """The plain-vs-FP32 workload: N = 100,000, phi = 0.2, K = 128, p = 6,
+    rc = 4, one right-hand side, the same xi (the one minimising the error
+    estimate at that cutoff) for both precisions.  Full operator against the
+    oracle's full M.f, and both against the committed direct-Ewald rows (at
+    rc = 4 the real-space truncation dominates: the estimate is 1.3e-2)."""
+    import os
+    from spme_amd import params as PR
+    _say("case: c2 N=100000 K=128 p=6 rc=4 fp64+fp32 vs oracle and direct")
+    cfg = W.CONFIGS["c2"]
+    N, K, p, rc = cfg["n"], cfg["K"], cfg["p"], cfg["rc"]
+    sy = W.make_system(cfg["seed"], N, cfg["phi"])
+    xi = 0.58
+    f = W.forces(cfg["seed"], N, 1).astype(np.float32).astype(np.float64)
+    ref = oracle.spme_full(xi, K, p, rc, sy.L, sy.pos, f)
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "c2.npz"))
+    rows, direct = g["rows"], g["direct"][0]
+    est = PR.estimate_error(xi, K, p, rc, cfg["phi"], sy.L)
+    errs = {}
+    for prec, tol, dt in [(E.SPME_FP64, FP64_TOL, torch.float64),
+                          (E.SPME_FP32, FP32_TOL, torch.float32)]:
+        eng = E.Engine(xi, K, p, N, sy.L, rc=rc, precision=prec)
+        eng.set_positions(torch.tensor(sy.pos, device="cuda"))
+        u = eng.apply(torch.tensor(f, device="cuda", dtype=dt), parts=E.SPME_FULL)
+        got = u.double().cpu().numpy()
+        eng.close()
+        assert _rel(got, ref) < tol, prec
+        mine = got[0].reshape(N, 3)[rows].reshape(-1)
+        errs[prec] = float(np.linalg.norm(mine - direct) / np.linalg.norm(direct))
+    _say(f"  error vs direct: fp64 {errs[E.SPME_FP64]:.3e}, fp32 {errs[E.SPME_FP32]:.3e}, "
+         f"estimate {est:.3e}")
+    assert abs(errs[E.SPME_FP32] - errs[E.SPME_FP64]) < 1e-4 * errs[E.SPME_FP64] + 1e-6
+    assert est / 2 < errs[E.SPME_FP64] < 2 * est
+
+
 @pytest.mark.parametrize("K,p", [(32, 4), (48, 6), (64, 8)])
 def test_recip_fp64_and_fp32(oracle, E, torch, K, p):
     _say(f"case: recip K={K} p={p} N=1500 fp64+fp32")
