@@ -171,8 +171,8 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
 // CC components (CC*sizeof(T) = 16 bytes: 4 FP32 or 2 FP64).  The tile and
 // its (P-1)-wide halo are staged in LDS with the CC components of a point
 // side by side, so one 16-byte LDS read gives a lane all its components.
-// The z range is swept in phases of S steps: a ring of S + P - 1 planes
-// holds what the particles of one phase need, and the S new planes and the
+// The z range is swept in phases of S steps: a ring of S + P - 1 = 16
+// planes holds what the particles of one phase need, and the S new planes and the
 // particles of the next phase are loaded into registers while the current
 // phase is processed (two barriers per phase).  Each particle is handled by
 // one wave, lanes = the P x P stencil columns; the per-lane products are
@@ -230,9 +230,12 @@ template <typename T> struct alignas(16) Vec16 {
   T v[N];
 };
 
-// Steps per phase and particles staged per batch.
+// Ring of planes (a power of two, so a plane's slot is z & (NSLOT-1)),
+// steps per phase (the ring holds a phase's S + P - 1 planes) and particles
+// staged per batch.
 template <int P> struct InterpPhase {
-  static constexpr int S = 8;
+  static constexpr int NSLOT = 16;
+  static constexpr int S = NSLOT - P + 1;
   static constexpr int MAXB = 128;
 };
 
@@ -247,9 +250,10 @@ __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
   static_assert(CC * sizeof(T) == 16, "16-byte component groups");
   using V = Vec16<T>;
   constexpr int S = InterpPhase<P>::S, MAXB = InterpPhase<P>::MAXB;
+  constexpr int NSLOT = InterpPhase<P>::NSLOT;
+  static_assert((NSLOT & (NSLOT - 1)) == 0 && S + P - 1 == NSLOT, "ring of planes");
   constexpr int E = TXY + P - 1;
   constexpr int PLANE = E * E;          // points per plane
-  constexpr int NSLOT = S + P - 1;      // ring of planes
   constexpr int NSUB = TXY / 8;         // sort tiles are 8 x 8
   constexpr int NS2 = NSUB * NSUB;
   constexpr int NWV = NT / 64;
@@ -296,7 +300,20 @@ __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
   // prologue: planes z0 .. z0+P-2
   for (int q = tid; q < (P - 1) * PLANE; q += NT) {
     const int j = q / PLANE, p = q - j * PLANE;
-    win[((z0 + j) % NSLOT) * PLANE + p] = load_point(z0 + j, p);
+    win[((z0 + j) & (NSLOT - 1)) * PLANE + p] = load_point(z0 + j, p);
+  }
+  // the phase loads of this thread: plane j_of[r] of the phase, point with
+  // in-plane offset src_off[r] (mesh) and pt_of[r] (window); fixed over the
+  // sweep, so computed once
+  long src_off[NPT];
+  int pt_of[NPT], j_of[NPT];
+#pragma unroll
+  for (int r = 0; r < NPT; ++r) {
+    const int q = tid + NT * r;
+    const int j = q / PLANE, p = q - j * PLANE;
+    j_of[r] = q < S * PLANE ? j : S;  // S: no load
+    pt_of[r] = p;
+    src_off[r] = src_of(p);
   }
 
   // registers carrying the next phase's loads
@@ -330,18 +347,20 @@ __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
     const int zfirst = zs + P - 1, zlast = min(zs + S, zend) + P - 1;  // [zfirst, zlast)
 #pragma unroll
     for (int r = 0; r < NPT; ++r) {
-      const int q = tid + NT * r;
-      const int j = q / PLANE, p = q - j * PLANE;
-      if (zfirst + j < zlast && q < S * PLANE) vp[r] = load_point(zfirst + j, p);
+      const int z = zfirst + j_of[r];
+      if (z < zlast) {
+        const T *g = g0 + (z >= K ? z - K : z) * zstride + src_off[r];
+#pragma unroll
+        for (int c = 0; c < CC; ++c) vp[r].v[c] = c < ncc ? g[(size_t)c * comp_stride] : T(0);
+      }
     }
   };
   auto commit_planes = [&](int zs) {
     const int zfirst = zs + P - 1, zlast = min(zs + S, zend) + P - 1;
 #pragma unroll
     for (int r = 0; r < NPT; ++r) {
-      const int q = tid + NT * r;
-      const int j = q / PLANE, p = q - j * PLANE;
-      if (zfirst + j < zlast && q < S * PLANE) win[((zfirst + j) % NSLOT) * PLANE + p] = vp[r];
+      const int z = zfirst + j_of[r];
+      if (z < zlast) win[(z & (NSLOT - 1)) * PLANE + pt_of[r]] = vp[r];
     }
   };
   auto fetch_particles = [&](int b0) {
@@ -381,15 +400,25 @@ __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
       T acc[CC];
 #pragma unroll
       for (int c = 0; c < CC; ++c) acc[c] = T(0);
-      int sl = bz % NSLOT;
+      const int sl0 = bz & (NSLOT - 1);
       const V *pt = win + (ry + dyc) * E + rx + dxc;
+      if (sl0 + P <= NSLOT) {  // the particle's planes sit in consecutive slots
+        const V *pz = pt + sl0 * PLANE;
 #pragma unroll
-      for (int j = 0; j < P; ++j) {
-        const T wz = w[2 * P + j];
-        const V v = pt[sl * PLANE];
+        for (int j = 0; j < P; ++j) {
+          const T wz = w[2 * P + j];
+          const V v = pz[j * PLANE];
 #pragma unroll
-        for (int c = 0; c < CC; ++c) acc[c] += wz * v.v[c];
-        sl = sl + 1 == NSLOT ? 0 : sl + 1;
+          for (int c = 0; c < CC; ++c) acc[c] += wz * v.v[c];
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+          const T wz = w[2 * P + j];
+          const V v = pt[((sl0 + j) & (NSLOT - 1)) * PLANE];
+#pragma unroll
+          for (int c = 0; c < CC; ++c) acc[c] += wz * v.v[c];
+        }
       }
 #pragma unroll
       for (int c = 0; c < CC; ++c) acc[c] *= wxy;
