@@ -21,7 +21,18 @@
 extern "C" {
 #endif
 
-enum { SPME_FP64 = 0, SPME_FP32 = 1 };
+/* Precision and data layout of the reciprocal path:
+ *   SPME_FP64        FP64, compact separable weights, sorted particles,
+ *                    batched right-hand sides, influence fused in the FFT;
+ *   SPME_FP32        the same in FP32 (tables and setup in FP64);
+ *   SPME_FP64_PLAIN  FP64 with the reference's data structures and call
+ *                    pattern (old-version/spme.c:158-324,560-726): a dense
+ *                    p^3 weight row and int32 mesh offsets per particle,
+ *                    spreading by atomic adds into a zeroed mesh, one
+ *                    right-hand side at a time, influence and 1/K^3 scale as
+ *                    passes of their own.  The yardstick for the others.
+ *                    No tiles, groups or debug stages. */
+enum { SPME_FP64 = 0, SPME_FP32 = 1, SPME_FP64_PLAIN = 2 };
 
 /* Parts of M to apply (bit mask). */
 enum { SPME_RECIP = 1, SPME_REAL = 2, SPME_SELF = 4, SPME_FULL = 7 };
@@ -33,7 +44,7 @@ typedef struct spme_params {
   int np;          /* number of particles N */
   double box_size; /* cube side L */
   double rc;       /* real-space cutoff (used by SPME_REAL) */
-  int precision;   /* SPME_FP64 or SPME_FP32 (mesh, spread, FFT, interpolate) */
+  int precision;   /* SPME_FP64, SPME_FP32 or SPME_FP64_PLAIN (see above) */
   int max_rhs;     /* largest batch of right-hand sides per apply */
 } spme_params_t;
 

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include "spme_kernels.h"
+#include "spme_launch.h"
 
 namespace spme {
 
@@ -608,8 +609,9 @@ __device__ __forceinline__ void fft_x_c2r_block(T *__restrict__ grid, int RS, lo
   }
 }
 
-// Complex FFT along y, in place.  Block = (x chunk of XW, plane z, comp).
-template <typename T, int K, int XW, int S, int NT>
+// Complex FFT along y (ZAX false: block = (x chunk of XW, plane z, comp)) or
+// along z (ZAX true: block = (x chunk of XW, row y, comp)), in place.
+template <typename T, int K, int XW, int S, int NT, bool ZAX = false>
 __device__ __forceinline__ void fft_y_block(T *__restrict__ grid, long comp_stride, int Khp,
                                             int nxc, const cpx<T> *__restrict__ tw, int lb,
                                             cpx<T> *buf, cpx<T> *s_tw) {
@@ -619,17 +621,19 @@ __device__ __forceinline__ void fft_y_block(T *__restrict__ grid, long comp_stri
   load_tw<T, K>(s_tw, tw);
   __syncthreads();
   const int xc = lb % nxc;
-  const int zc = lb / nxc;  // z + K*comp
+  const int zc = lb / nxc;  // (z, or y along z) + K*comp
   const int comp = zc / K, z = zc - comp * K;
   const int x0 = xc * XW;
-  cpx<T> *plane = reinterpret_cast<cpx<T> *>(grid + comp * comp_stride) + (size_t)z * K * Khp;
+  cpx<T> *plane = reinterpret_cast<cpx<T> *>(grid + comp * comp_stride) +
+                  (ZAX ? (size_t)z * Khp : (size_t)z * K * Khp);
+  const size_t cstr = ZAX ? (size_t)K * Khp : (size_t)Khp;  // stride of the transformed index
   for (int t = threadIdx.x; t < XW * N2; t += NT) {  // step A from global
     const int xo = t % XW, n2 = t / XW;
     const int x = x0 + xo;
     cpx<T> a[N1];
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1)
-      a[n1] = x < Kh ? plane[(size_t)(n1 * N2 + n2) * Khp + x] : cpx<T>{T(0), T(0)};
+      a[n1] = x < Kh ? plane[(size_t)(n1 * N2 + n2) * cstr + x] : cpx<T>{T(0), T(0)};
     reg_fft<N1, S>(a);
 #pragma unroll
     for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<S>(s_tw, n2 * k1));
@@ -646,42 +650,48 @@ __device__ __forceinline__ void fft_y_block(T *__restrict__ grid, long comp_stri
     reg_fft<N2, S>(b);
     if (x < Kh) {
 #pragma unroll
-      for (int k2 = 0; k2 < N2; ++k2) plane[(size_t)(k1 + N1 * k2) * Khp + x] = b[k2];
+      for (int k2 = 0; k2 < N2; ++k2) plane[(size_t)(k1 + N1 * k2) * cstr + x] = b[k2];
     }
   }
 }
 
 // Forward FFT along z, RPY influence (old-version/spme_kernel.c:73-228),
 // backward FFT along z, for the 3 components of one right-hand side.
-// Block = (x chunk, y) x rhs.  tab[(z*K + y)*Khp + x] = psi(|k|)|b(k)|^2/V.
+// Blocks follow ZLayout: per right-hand side, (x chunk, y) blocks of XW
+// adjacent columns, then (when K/2 is a multiple of XW) Nyquist blocks whose
+// XW lanes take column x = K/2 of XW consecutive rows y.
+// tab[(z*K + y)*Khp + x] = psi(|k|)|b(k)|^2/V.
 // Wave numbers follow old-version/spme.c:95-107 (index K/2 -> +pi K/L).
 // The spectrum of a (column, k1) stays in registers between the forward
 // N2-point FFT, the 3x3 influence and the backward N2-point FFT.
 template <typename T, int K, int XW, int NT>
 __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp_stride,
-                                                 int Khp, int nxc, const T *__restrict__ tab,
-                                                 T dk, const cpx<T> *__restrict__ tw, int lb,
-                                                 cpx<T> *buf, cpx<T> *s_tw, T *s_tab) {
+                                                 int Khp, const T *__restrict__ tab, T dk,
+                                                 const cpx<T> *__restrict__ tw, int lb,
+                                                 cpx<T> *buf, cpx<T> *s_tw) {
   constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
   constexpr int Kh = K / 2 + 1;
   constexpr int LS = K + 1;
+  using Z = ZLayout<T>;
+  static_assert(Z::XW == XW, "z pass width");
+  static_assert(XW * N1 <= NT, "one step-B column per thread");
+  constexpr int NXC = Z::chunks(K), PER = Z::per_rhs(K);
   load_tw<T, K>(s_tw, tw);
-  // adjacent x chunks (halves of one cache line) on the same XCD
-  const int xc = lb % nxc, y = (lb / nxc) % K;
-  const int x0 = xc * XW;
-  const int rhs = lb / (nxc * K);
-  for (int t = threadIdx.x; t < XW * K; t += NT) {
-    const int z = t / XW, xo = t - z * XW;
-    const int x = x0 + xo;
-    s_tab[xo * K + z] = x < Kh ? tab[((size_t)z * K + y) * Khp + x] : T(0);
+  __syncthreads();  // step A reads twiddles that other threads loaded
+  const int rhs = lb / PER, j = lb - rhs * PER;
+  // a lane's column (x, y) is (xa + xo*dx, ya + xo*dy)
+  int xa, ya, dx, dy;
+  if (j < NXC * K) {  // adjacent x chunks (halves of one cache line) on the same XCD
+    xa = (j % NXC) * XW, ya = j / NXC, dx = 1, dy = 0;
+  } else {
+    xa = K / 2, ya = (j - NXC * K) * XW, dx = 0, dy = 1;
   }
-  __syncthreads();
   cpx<T> *g0 = reinterpret_cast<cpx<T> *>(grid + (3L * rhs) * comp_stride);
   const long cs = comp_stride / 2;  // complex elements
   for (int t = threadIdx.x; t < 3 * XW * N2; t += NT) {  // step A (forward) from global
     const int xo = t % XW, r = t / XW;
     const int c = r % 3, n2 = r / 3;
-    const int x = x0 + xo;
+    const int x = xa + xo * dx, y = ya + xo * dy;
     cpx<T> a[N1];
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1)
@@ -694,10 +704,14 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
     for (int k1 = 0; k1 < N1; ++k1) buf[(c * XW + xo) * LS + k1 * N2 + n2] = a[k1];
   }
   __syncthreads();
-  const T ky = T(y <= K / 2 ? y : y - K) * dk;
-  for (int t = threadIdx.x; t < XW * N1; t += NT) {  // step B, influence, inverse step B
-    const int xo = t % XW, k1 = t / XW;
-    const int x = x0 + xo;
+  if (threadIdx.x < XW * N1) {  // step B, influence, inverse step B
+    const int xo = threadIdx.x % XW, k1 = threadIdx.x / XW;
+    const int x = xa + xo * dx, y = ya + xo * dy;
+    // k = 0 and padding columns carry m2 = 0
+    T m2[N2];
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2)
+      m2[k2] = x < Kh ? tab[((size_t)(k1 + N1 * k2) * K + y) * Khp + x] : T(0);
     cpx<T> b[3][N2];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -706,20 +720,27 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
       reg_fft<N2, -1>(b[c]);
     }
     const T kx = T(x) * dk;  // x <= K/2 in the half spectrum
+    const T ky = T(y <= K / 2 ? y : y - K) * dk;
 #pragma unroll
     for (int k2 = 0; k2 < N2; ++k2) {
+      // every fused multiply-add is written out, so the single-system and
+      // grouped kernels round alike whatever their surrounding code
+#pragma clang fp contract(off)
       const int z = k1 + N1 * k2;
       const T kz = T(z <= K / 2 ? z : z - K) * dk;
-      const T k2n = kx * kx + ky * ky + kz * kz;
-      const T m2 = s_tab[xo * K + z];
-      // k = 0 and padding columns carry m2 = 0; guard the division there
-      const T ik = k2n > T(0) ? T(1) / k2n : T(0);
-      const T B0 = m2 - m2 * kx * kx * ik, B1 = -m2 * kx * ky * ik, B2 = -m2 * kx * kz * ik;
-      const T B3 = m2 - m2 * ky * ky * ik, B4 = -m2 * ky * kz * ik, B5 = m2 - m2 * kz * kz * ik;
+      const T k2n = fma(kz, kz, fma(ky, ky, kx * kx));
+      // guard the division at k = 0
+      const T mi = k2n > T(0) ? m2[k2] / k2n : T(0);
+      const T ax = mi * kx, ay = mi * ky, az = mi * kz;
+      const T B0 = fma(-ax, kx, m2[k2]), B1 = -ax * ky, B2 = -ax * kz;
+      const T B3 = fma(-ay, ky, m2[k2]), B4 = -ay * kz, B5 = fma(-az, kz, m2[k2]);
       const cpx<T> va = b[0][k2], vb = b[1][k2], vc = b[2][k2];
-      b[0][k2] = {B0 * va.x + B1 * vb.x + B2 * vc.x, B0 * va.y + B1 * vb.y + B2 * vc.y};
-      b[1][k2] = {B1 * va.x + B3 * vb.x + B4 * vc.x, B1 * va.y + B3 * vb.y + B4 * vc.y};
-      b[2][k2] = {B2 * va.x + B4 * vb.x + B5 * vc.x, B2 * va.y + B4 * vb.y + B5 * vc.y};
+      b[0][k2] = {fma(B2, vc.x, fma(B1, vb.x, B0 * va.x)),
+                  fma(B2, vc.y, fma(B1, vb.y, B0 * va.y))};
+      b[1][k2] = {fma(B4, vc.x, fma(B3, vb.x, B1 * va.x)),
+                  fma(B4, vc.y, fma(B3, vb.y, B1 * va.y))};
+      b[2][k2] = {fma(B5, vc.x, fma(B4, vb.x, B2 * va.x)),
+                  fma(B5, vc.y, fma(B4, vb.y, B2 * va.y))};
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -734,7 +755,7 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
   for (int t = threadIdx.x; t < 3 * XW * N2; t += NT) {  // inverse step A to global
     const int xo = t % XW, r = t / XW;
     const int c = r % 3, n2 = r / 3;
-    const int x = x0 + xo;
+    const int x = xa + xo * dx, y = ya + xo * dy;
     cpx<T> a[N1];
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) a[k1] = buf[(c * XW + xo) * LS + k1 * N2 + n2];

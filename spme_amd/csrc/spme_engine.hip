@@ -122,6 +122,7 @@ struct StageEvents {
 struct _spme_t {
   spme_params_t prm;
   int K, P, N, prec, max_rhs, Kh, Khp, RS;
+  bool dense;  // SPME_FP64_PLAIN: the reference's data structures
   long comp_stride;  // reals
   size_t tsz;        // bytes per real
   hipStream_t stream;
@@ -138,6 +139,10 @@ struct _spme_t {
   double *w_tmp;
   void *pw;  // sorted weights [N][3][P]
   void *fs;  // sorted forces [N][3*max_rhs]
+  // plain path: dense p^3 weights and int32 mesh offsets per particle
+  int ldP, ldind;
+  double *dP;
+  int *dind;
   // real space
   int nc;
   int *r_key, *r_counts, *r_cursor, *r_off, *r_perm;
@@ -176,8 +181,10 @@ int upload_tables(_spme_t *e) {
   HIP_TRY(hipMalloc(&d_bsq, sizeof(double) * K));
   HIP_TRY(hipMemcpy(d_bsq, bsq.data(), sizeof(double) * K, hipMemcpyHostToDevice));
   HIP_TRY(hipMalloc(&e->tab, sizeof(T) * (size_t)K * K * e->Khp));
-  launch_influence_table<T>(d_bsq, K, e->Khp, 2.0 * M_PI / L, xi, 1.0 / V, (T *)e->tab,
-                            e->stream);
+  // the plain path keeps the reference's lm2 = psi |b|^2 and scales by
+  // 1/K^3 and alpha K^3/V in passes of their own; the others fold 1/V here
+  launch_influence_table<T>(d_bsq, K, e->Khp, 2.0 * M_PI / L, xi, e->dense ? 1.0 : 1.0 / V,
+                            (T *)e->tab, e->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipFree(d_bsq));
@@ -205,8 +212,9 @@ int validate(const spme_params_t *p) {
   if (!(p->box_size > 0.0) || !std::isfinite(p->box_size))
     return fail(-1, "box_size must be positive");
   if (!(p->xi > 0.0) || !std::isfinite(p->xi)) return fail(-1, "xi must be positive");
-  if (p->precision != SPME_FP64 && p->precision != SPME_FP32)
-    return fail(-1, "precision must be SPME_FP64 or SPME_FP32");
+  if (p->precision != SPME_FP64 && p->precision != SPME_FP32 &&
+      p->precision != SPME_FP64_PLAIN)
+    return fail(-1, "precision must be SPME_FP64, SPME_FP32 or SPME_FP64_PLAIN");
   if (p->max_rhs < 1 || p->max_rhs > 16) return fail(-1, "max_rhs must be in [1, 16]");
   return 0;
 }
@@ -216,7 +224,7 @@ void free_all(_spme_t *e) {
                   e->cursor, e->boff, e->perm, e->pbase, e->err, e->w_tmp, e->pw, e->fs,
                   e->r_key, e->r_counts, e->r_cursor, e->r_off, e->r_perm, e->r_pos,
                   e->r_fs, e->r_cnt, e->r_width, e->r_size, e->r_soff, e->r_col, e->r_blk,
-                  e->h_in, e->h_out};
+                  e->h_in, e->h_out, e->dP, e->dind};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &r : e->pending) {
@@ -235,6 +243,7 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
   e->P = prm->porder;
   e->N = prm->np;
   e->prec = prm->precision;
+  e->dense = prm->precision == SPME_FP64_PLAIN;
   e->max_rhs = prm->max_rhs;
   e->Kh = e->K / 2 + 1;
   // rows padded to whole 128-byte lines in FP32 (16 complex) so that the
@@ -258,7 +267,8 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
   E_TRY(hipGetDevice(&e->device));
   E_TRY(hipDeviceGetAttribute(&e->ncu, hipDeviceAttributeMultiprocessorCount, e->device));
   E_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
-  E_TRY(hipMalloc(&e->grid, (size_t)C * e->comp_stride * e->tsz));
+  // the plain path transforms one right-hand side (3 components) at a time
+  E_TRY(hipMalloc(&e->grid, (size_t)(e->dense ? 3 : C) * e->comp_stride * e->tsz));
   int rc = e->prec == SPME_FP32 ? upload_tables<float>(e) : upload_tables<double>(e);
   if (rc) return bail(rc);
   e->nkeys = (K / 8) * (K / 8) * K;
@@ -272,8 +282,17 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
   E_TRY(hipMalloc(&e->pbase, sizeof(int) * N));
   E_TRY(hipMalloc(&e->err, sizeof(int)));
   E_TRY(hipMalloc(&e->w_tmp, sizeof(double) * 3 * e->P * N));
-  E_TRY(hipMalloc(&e->pw, e->tsz * 3 * e->P * N));
-  E_TRY(hipMalloc(&e->fs, e->tsz * (size_t)C * N));
+  if (e->dense) {
+    // PAD_LEN(p^3, 8) and PAD_LEN(p^3, 4) (old-version/spme.c:457-460)
+    const int p3 = e->P * e->P * e->P;
+    e->ldP = (p3 + 7) / 8 * 8;
+    e->ldind = (p3 + 15) / 16 * 16;
+    E_TRY(hipMalloc(&e->dP, sizeof(double) * (size_t)e->ldP * N));
+    E_TRY(hipMalloc(&e->dind, sizeof(int) * (size_t)e->ldind * N));
+  } else {
+    E_TRY(hipMalloc(&e->pw, e->tsz * 3 * e->P * N));
+    E_TRY(hipMalloc(&e->fs, e->tsz * (size_t)C * N));
+  }
   const double L = prm->box_size;
   // Real-space cell list only when a cutoff is given (the drop-in surface is
   // reciprocal-only).  Cells are at least max(rc, 2) wide so that every pair
@@ -311,10 +330,16 @@ int build_P(_spme_t *e, hipStream_t st) {
   HIP_TRY(hipMemsetAsync(e->err, 0, sizeof(int), st));
   launch_pbuild(e->pos, N, e->prm.box_size, K, P, W, 8, 8, e->base_tmp, e->w_tmp, e->key_tmp,
                 e->counts, e->err, st);
-  launch_scan(e->counts, e->boff, e->nkeys, st);
-  launch_pscatter(e->key_tmp, e->boff, e->cursor, e->perm, N, st);
-  launch_bucket_sort(e->boff, e->perm, e->nkeys, st);
-  launch_pgather<T>(e->perm, e->base_tmp, e->w_tmp, e->pbase, (T *)e->pw, N, 3 * P, st);
+  if (e->dense) {
+    if (launch_dense_P(e->base_tmp, e->w_tmp, N, K, e->RS, P, e->ldP, e->ldind, e->dP, e->dind,
+                       st))
+      return fail(-1, "dense P: unsupported porder");
+  } else {
+    launch_scan(e->counts, e->boff, e->nkeys, st);
+    launch_pscatter(e->key_tmp, e->boff, e->cursor, e->perm, N, st);
+    launch_bucket_sort(e->boff, e->perm, e->nkeys, st);
+    launch_pgather<T>(e->perm, e->base_tmp, e->w_tmp, e->pbase, (T *)e->pw, N, 3 * P, st);
+  }
   if (e->nc >= 3) {
     const int nc3 = e->nc * e->nc * e->nc;
     HIP_TRY(hipMemsetAsync(e->r_counts, 0, sizeof(int) * nc3, st));
@@ -451,8 +476,38 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
     rec.marks.push_back({stage, ev});
     return 0;
   };
+  if ((parts & SPME_RECIP) && e->dense) {
+    if (stop_after) return fail(-1, "debug stages are not available on the plain path");
+    // one right-hand side at a time (old-version/spme.c:631-702)
+    const int K = e->K;
+    const long cs = e->comp_stride;
+    const double L = e->prm.box_size;
+    const double *fin = reinterpret_cast<const double *>(in);
+    double *uout = reinterpret_cast<double *>(out);
+    double *grid = (double *)e->grid;
+    FftArgs<double> fa{grid, cs, e->RS, K, 3, (const double *)e->tab, 2.0 * M_PI / L,
+                       (const cpx<double> *)e->tw, e->ncu};
+    const double k3 = double(K) * K * K;
+    for (int r = 0; r < nrhs; ++r) {
+      HIP_TRY(hipMemsetAsync(grid, 0, sizeof(double) * 3 * cs, st));
+      if (launch_dense_spread(e->dP, e->dind, N, e->P, e->ldP, e->ldind, fin + r * ldin, grid,
+                              cs, st))
+        return fail(-1, "spread: unsupported porder");
+      mark(SPME_ST_SPREAD);
+      if (launch_fft_plain<double>(fa, -1, st)) return fail(-1, "fft: unsupported size");
+      mark(SPME_ST_FFT_X_FWD);
+      launch_dense_influence(grid, cs, K, e->Khp, (const double *)e->tab, 2.0 * M_PI / L, st);
+      mark(SPME_ST_FFT_Z_INFL);
+      launch_fft_plain<double>(fa, +1, st);
+      launch_dense_scale(grid, 3 * cs, 1.0 / k3, st);
+      mark(SPME_ST_FFT_X_INV);
+      launch_dense_interp(e->dP, e->dind, N, e->P, e->ldP, e->ldind, grid, cs,
+                          alpha * k3 / (L * L * L), beta, uout + r * ldout, st);
+      mark(SPME_ST_INTERP);
+    }
+  }
   const Tiles tl = resolve_tiles(e, e->tiles, C);
-  if (parts & SPME_RECIP) {
+  if ((parts & SPME_RECIP) && !e->dense) {
     launch_permute_in<T>(e->perm, in, ldin, (T *)e->fs, N, C, st);
     mark(SPME_ST_PERMUTE);
     SpreadArgs<T> sa{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, C, tl.spread_cc,
@@ -1085,6 +1140,7 @@ int spme_engine_debug_stage(spme_t *e, int stage, int nrhs, const void *in, long
                             void *host_mesh) {
   if (!e) return fail(-1, "null engine");
   if (stage != 1 && stage != 2) return fail(-1, "stage must be 1 or 2");
+  if (e->dense) return fail(-1, "debug stages are not available on the plain path");
   int rc = apply(e, nrhs, 1.0, in, ldin, 0.0, (void *)in, ldin, SPME_RECIP, e->stream, stage);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(host_mesh, e->grid, (size_t)3 * nrhs * e->comp_stride * e->tsz,
@@ -1126,6 +1182,7 @@ void cutoff2xi(double rmax, double ewald_eps, double box_size, double *_xi) {
 
 int spme_engine_set_tiles(spme_t *spme, const spme_tiles_t *tiles) {
   if (!spme) return fail(-1, "null engine");
+  if (spme->dense) return fail(-1, "the plain path has no tile shapes");
   const spme_tiles_t t = tiles ? *tiles : spme_tiles_t{};
   if (int rc = check_tiles(t)) return rc;
   spme->tiles = t;
@@ -1134,6 +1191,7 @@ int spme_engine_set_tiles(spme_t *spme, const spme_tiles_t *tiles) {
 
 int spme_engine_get_tiles(const spme_t *spme, int nrhs, spme_tiles_t *tiles) {
   if (!spme || !tiles) return fail(-1, "null argument");
+  if (spme->dense) return fail(-1, "the plain path has no tile shapes");
   if (nrhs < 1 || nrhs > spme->max_rhs)
     return fail(-1, "nrhs must be in [1, %d] (got %d)", spme->max_rhs, nrhs);
   const Tiles r = resolve_tiles(spme, spme->tiles, 3 * nrhs);
@@ -1146,6 +1204,7 @@ int spme_engine_get_tiles(const spme_t *spme, int nrhs, spme_tiles_t *tiles) {
 
 int spme_engine_autotune(spme_t *spme, int nrhs, int reps, spme_tiles_t *best, void *stream) {
   if (!spme) return fail(-1, "null engine");
+  if (spme->dense) return fail(-1, "the plain path has no tile shapes");
   (void)hipGetLastError();
   if (!spme->have_pos) return fail(-1, "positions have not been set");
   if (nrhs < 1 || nrhs > spme->max_rhs)
@@ -1165,6 +1224,7 @@ int spme_group_create(int nsys, spme_t *const *engines, int nrhs, spme_group_t *
   for (int s = 0; s < nsys; ++s) {
     const spme_t *e = engines[s];
     if (!e) return fail(-1, "engine %d is null", s);
+    if (e->dense) return fail(-1, "engine %d: plain-path engines cannot be grouped", s);
     if (e->prec != engines[0]->prec || e->P != engines[0]->P)
       return fail(-1, "engine %d: all engines need one precision and one porder", s);
     if (nrhs < 1 || nrhs > e->max_rhs)

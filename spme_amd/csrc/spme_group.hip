@@ -174,7 +174,6 @@ __global__ __launch_bounds__(NT) void kg_fft_z_infl(GroupStage<FftArgs<T>> t) {
   constexpr int XW = FftShape<T>::XWZ;
   __shared__ cpx<T> buf[3 * XW * (KMAX + 1)];
   __shared__ cpx<T> s_tw[KMAX];
-  __shared__ T s_tab[XW * KMAX];
   {
     const int g = blockIdx.x;
     if (g >= t.total) return;
@@ -182,13 +181,12 @@ __global__ __launch_bounds__(NT) void kg_fft_z_infl(GroupStage<FftArgs<T>> t) {
     const int s = group_locate(t, g, true, lb);
     const FftArgs<T> &a = t.args[s];
     const int Khp = a.RS / 2;
-    const int nxc = (a.K / 2 + 1 + XW - 1) / XW;
     switch (a.K) {
-#define X(KK)                                                                             \
-  case KK:                                                                                \
-    if constexpr (KK <= KMAX)                                                             \
-      fft_z_infl_block<T, KK, XW, NT>(a.grid, a.comp_stride, Khp, nxc, a.tab, a.dk, a.tw, \
-                                      lb, buf, s_tw, s_tab);                              \
+#define X(KK)                                                                                \
+  case KK:                                                                                   \
+    if constexpr (KK <= KMAX)                                                                \
+      fft_z_infl_block<T, KK, XW, NT>(a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw, lb, buf, \
+                                      s_tw);                                                 \
     break;
       SPME_FOR_EACH_K(X)
 #undef X

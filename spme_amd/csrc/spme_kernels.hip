@@ -186,15 +186,24 @@ __global__ __launch_bounds__(NT) void k_fft_y(T *__restrict__ grid, long comp_st
                                buf, s_tw);
 }
 
+// FFT along z alone (the plain path applies the influence in a pass of its own)
+template <typename T, int K, int XW, int S, int NT>
+__global__ __launch_bounds__(NT) void k_fft_z(T *__restrict__ grid, long comp_stride, int Khp,
+                                              int nxc, const cpx<T> *__restrict__ tw) {
+  __shared__ cpx<T> buf[XW * (K + 1)];
+  __shared__ cpx<T> s_tw[K];
+  fft_y_block<T, K, XW, S, NT, true>(grid, comp_stride, Khp, nxc, tw,
+                                     xcd_remap(blockIdx.x, gridDim.x), buf, s_tw);
+}
+
 template <typename T, int K, int XW, int NT>
 __global__ __launch_bounds__(NT) void k_fft_z_infl(T *__restrict__ grid, long comp_stride,
-                                                   int Khp, int nxc, const T *__restrict__ tab,
-                                                   T dk, const cpx<T> *__restrict__ tw) {
+                                                   int Khp, const T *__restrict__ tab, T dk,
+                                                   const cpx<T> *__restrict__ tw) {
   __shared__ cpx<T> buf[3 * XW * (K + 1)];
   __shared__ cpx<T> s_tw[K];
-  __shared__ T s_tab[XW * K];
-  fft_z_infl_block<T, K, XW, NT>(grid, comp_stride, Khp, nxc, tab, dk, tw,
-                                 xcd_remap(blockIdx.x, gridDim.x), buf, s_tw, s_tab);
+  fft_z_infl_block<T, K, XW, NT>(grid, comp_stride, Khp, tab, dk, tw,
+                                 xcd_remap(blockIdx.x, gridDim.x), buf, s_tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -678,7 +687,7 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
   const long npairs = rows / 2;
   const int Khp = a.RS / 2;
   const int nxc = (K / 2 + 1 + S::XW - 1) / S::XW;
-  const int nxcz = (K / 2 + 1 + S::XWZ - 1) / S::XWZ;
+  const int nzb = ZLayout<T>::per_rhs(K) * (a.ncomp / 3);
   auto mark = [&](int pass) {
     if (mk) {
       (void)hipEventRecord(mk->ev[mk->n], st);
@@ -693,8 +702,8 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
     hipLaunchKernelGGL((k_fft_xy_fwd<T, K, NTXY>), dim3(grid), dim3(NTXY), 0, st, a.grid, a.RS,
                        a.comp_stride, nplanes, a.tw);
     mark(0);
-    hipLaunchKernelGGL((k_fft_z_infl<T, K, S::XWZ, NT>), dim3(nxcz * K * (a.ncomp / 3)),
-                       dim3(NT), 0, st, a.grid, a.comp_stride, Khp, nxcz, a.tab, a.dk, a.tw);
+    hipLaunchKernelGGL((k_fft_z_infl<T, K, S::XWZ, NT>), dim3(nzb), dim3(NT), 0, st, a.grid,
+                       a.comp_stride, Khp, a.tab, a.dk, a.tw);
     mark(2);
     hipLaunchKernelGGL((k_fft_xy_inv<T, K, NTXY>), dim3(grid), dim3(NTXY), 0, st, a.grid, a.RS,
                        a.comp_stride, nplanes, a.tw);
@@ -706,8 +715,8 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
     hipLaunchKernelGGL((k_fft_y<T, K, S::XW, -1, NT>), dim3(nxc * K * a.ncomp), dim3(NT), 0,
                        st, a.grid, a.comp_stride, Khp, nxc, a.tw);
     mark(1);
-    hipLaunchKernelGGL((k_fft_z_infl<T, K, S::XWZ, NT>), dim3(nxcz * K * (a.ncomp / 3)),
-                       dim3(NT), 0, st, a.grid, a.comp_stride, Khp, nxcz, a.tab, a.dk, a.tw);
+    hipLaunchKernelGGL((k_fft_z_infl<T, K, S::XWZ, NT>), dim3(nzb), dim3(NT), 0, st, a.grid,
+                       a.comp_stride, Khp, a.tab, a.dk, a.tw);
     mark(2);
     hipLaunchKernelGGL((k_fft_y<T, K, S::XW, +1, NT>), dim3(nxc * K * a.ncomp), dim3(NT), 0,
                        st, a.grid, a.comp_stride, Khp, nxc, a.tw);
@@ -733,6 +742,50 @@ int launch_fft_influence(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
   }
   return -1;
 }
+
+// Forward (dir -1: x r2c, y, z) or backward (dir +1: z, y, x c2r) transform
+// of a.ncomp components, unnormalised, no influence.
+template <typename T, int K>
+static void fft3d_plain_k(const FftArgs<T> &a, int dir, hipStream_t st) {
+  using S = FftShape<T>;
+  constexpr int NT = 256;
+  const long npairs = (long)a.ncomp * K * K / 2;
+  const int Khp = a.RS / 2;
+  const int nxc = (K / 2 + 1 + S::XW - 1) / S::XW;
+  const dim3 gyz(nxc * K * a.ncomp);
+  if (dir < 0) {
+    hipLaunchKernelGGL((k_fft_x_r2c<T, K, S::NLX, NT>), dim3(grid1(npairs, S::NLX)), dim3(NT),
+                       0, st, a.grid, a.RS, npairs, a.tw);
+    hipLaunchKernelGGL((k_fft_y<T, K, S::XW, -1, NT>), gyz, dim3(NT), 0, st, a.grid,
+                       a.comp_stride, Khp, nxc, a.tw);
+    hipLaunchKernelGGL((k_fft_z<T, K, S::XW, -1, NT>), gyz, dim3(NT), 0, st, a.grid,
+                       a.comp_stride, Khp, nxc, a.tw);
+  } else {
+    hipLaunchKernelGGL((k_fft_z<T, K, S::XW, +1, NT>), gyz, dim3(NT), 0, st, a.grid,
+                       a.comp_stride, Khp, nxc, a.tw);
+    hipLaunchKernelGGL((k_fft_y<T, K, S::XW, +1, NT>), gyz, dim3(NT), 0, st, a.grid,
+                       a.comp_stride, Khp, nxc, a.tw);
+    hipLaunchKernelGGL((k_fft_x_c2r<T, K, S::NLX, NT>), dim3(grid1(npairs, S::NLX)), dim3(NT),
+                       0, st, a.grid, a.RS, npairs, a.tw);
+  }
+}
+
+template <typename T>
+int launch_fft_plain(const FftArgs<T> &a, int dir, hipStream_t st) {
+  switch (a.K) {
+    case 16: fft3d_plain_k<T, 16>(a, dir, st); return 0;
+    case 24: fft3d_plain_k<T, 24>(a, dir, st); return 0;
+    case 32: fft3d_plain_k<T, 32>(a, dir, st); return 0;
+    case 48: fft3d_plain_k<T, 48>(a, dir, st); return 0;
+    case 64: fft3d_plain_k<T, 64>(a, dir, st); return 0;
+    case 96: fft3d_plain_k<T, 96>(a, dir, st); return 0;
+    case 128: fft3d_plain_k<T, 128>(a, dir, st); return 0;
+    case 192: fft3d_plain_k<T, 192>(a, dir, st); return 0;
+    case 256: fft3d_plain_k<T, 256>(a, dir, st); return 0;
+  }
+  return -1;
+}
+template int launch_fft_plain<double>(const FftArgs<double> &, int, hipStream_t);
 
 bool fft_size_supported(int K) {
   return K == 16 || K == 24 || K == 32 || K == 48 || K == 64 || K == 96 || K == 128 ||

@@ -57,6 +57,21 @@ template <typename T> struct FftShape {
   static constexpr int XWZ = sizeof(T) == 8 ? 4 : 8;   // z-pass columns (x3 comps)
 };
 
+// Blocks of the z pass for one right-hand side (3 components): chunks of XWZ
+// x columns for every y.  When K/2 is a multiple of XWZ the chunks cover
+// x < K/2 and the Nyquist column x = K/2 gets K/XWZ narrow blocks of XWZ
+// rows of y each, instead of a chunk that is one column wide.
+template <typename T> struct ZLayout {
+  static constexpr int XW = FftShape<T>::XWZ;
+  __host__ __device__ static constexpr bool nyq(int K) { return (K / 2) % XW == 0; }
+  __host__ __device__ static constexpr int chunks(int K) {
+    return nyq(K) ? K / 2 / XW : (K / 2 + XW) / XW;
+  }
+  __host__ __device__ static constexpr int per_rhs(int K) {
+    return chunks(K) * K + (nyq(K) ? K / XW : 0);
+  }
+};
+
 template <typename T> struct FftArgs {
   T *grid;
   long comp_stride;
@@ -94,6 +109,21 @@ template <typename T> int launch_interp(const InterpArgs<T> &a, hipStream_t st);
 // mk (optional): events to record after each launch (mk->n = 0 on entry).
 template <typename T>
 int launch_fft_influence(const FftArgs<T> &a, hipStream_t st, FftMarks *mk = nullptr);
+// Unnormalised 3-D transform without influence: dir -1 forward (x r2c, y, z),
+// dir +1 backward (z, y, x c2r).
+template <typename T> int launch_fft_plain(const FftArgs<T> &a, int dir, hipStream_t st);
+
+// Plain FP64 path with the reference's data structures (spme_dense.hip).
+int launch_dense_P(const int *base, const double *w, int N, int K, int RS, int P, int ldP,
+                   int ldind, double *Pm, int *ind, hipStream_t st);
+int launch_dense_spread(const double *Pm, const int *ind, int N, int P, int ldP, int ldind,
+                        const double *f, double *grid, long cs, hipStream_t st);
+void launch_dense_influence(double *grid, long cs, int K, int Khp, const double *lm2,
+                            double dk, hipStream_t st);
+void launch_dense_scale(double *grid, long n, double s, hipStream_t st);
+int launch_dense_interp(const double *Pm, const int *ind, int N, int P, int ldP, int ldind,
+                        const double *grid, long cs, double alpha, double beta, double *u,
+                        hipStream_t st);
 bool fft_size_supported(int K);
 template <typename T>
 void launch_influence_table(const double *bsq, int K, int Khp, double dk, double xi,

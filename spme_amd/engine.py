@@ -16,12 +16,13 @@ import ctypes
 import numpy as np
 
 from . import lib as _L
-from .lib import (SPME_FP32, SPME_FP64, SPME_FULL, SPME_REAL, SPME_RECIP,
-                  SPME_SELF, SpmeError, SpmeInfo, SpmeParams, SpmeTiles, check,
-                  load)
+from .lib import (SPME_FP32, SPME_FP64, SPME_FP64_PLAIN, SPME_FULL, SPME_REAL,
+                  SPME_RECIP, SPME_SELF, SpmeError, SpmeInfo, SpmeParams, SpmeTiles,
+                  check, load)
 
 __all__ = ["Engine", "Group", "DropIn", "cutoff2xi", "SPME_FP32", "SPME_FP64",
-           "SPME_FULL", "SPME_REAL", "SPME_RECIP", "SPME_SELF", "SpmeError"]
+           "SPME_FP64_PLAIN", "SPME_FULL", "SPME_REAL", "SPME_RECIP", "SPME_SELF",
+           "SpmeError"]
 
 
 def cutoff2xi(rmax: float, ewald_eps: float, box_size: float) -> float:
