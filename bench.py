@@ -1,8 +1,13 @@
 """Benchmark: RPY mobility products u = M.f per second at N = 200,000,
 K = 192^3, p = 8, 16 right-hand sides in FP32 (block-Lanczos width), beside
-the plain FP64 path (one right-hand side per apply, as the reference's
-compute_spme loops them, old-version/spme.c:631-702) on the same positions
-and parameters.
+two FP64 paths on the same positions and parameters, one right-hand side
+per apply as the reference's compute_spme loops them
+(old-version/spme.c:631-702):
+  - plain: the reference's data structures (dense p^3 weights and int32
+    offsets per particle, atomic spreading, separate influence and 1/K^3
+    scale passes), SPME_FP64_PLAIN;
+  - compact: this engine's FP64 path (separable weights, sorted particles,
+    fused influence), SPME_FP64.
 
 A step is one batched apply of the full operator (wave-space SPME + real
 space + overlap + self) to 16 force vectors = 16 M.f.  Inputs are resident
@@ -147,6 +152,29 @@ def traffic_per_launch(kernel):
     return None
 
 
+def hbm_copy_ceiling(gib=2, reps=10):
+    """Device-to-device copy of a 2 GiB buffer (8x the 256 MB Infinity Cache):
+    read + write bytes per second, the practical ceiling of a streaming
+    kernel next to the published 8 TB/s."""
+    import torch
+    n = gib * (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2.0 * n * 4 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return {"GBs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4),
+            "what": f"{gib} GiB device copy, read + write bytes"}
+
+
 def multi_system(steps, warmup):
     """64 uneven systems: separate applies vs grouped launches vs one graph."""
     import torch
@@ -271,29 +299,42 @@ def main():
     u_full = u32_d.double().cpu().numpy()
     eng.close()
 
-    # ---- plain FP64 path, one right-hand side per apply ---------------------
+    # ---- FP64 paths, one right-hand side per apply ---------------------------
     plain_steps = args.plain_steps or max(2, args.steps // 2)
-    engp = E.Engine(xi, K, p, N, L, rc=rc, precision=E.SPME_FP64, max_rhs=1)
-    engp.set_positions(pos_d)
     f64_d = torch.tensor(f64, device=dev)
     u64_d = torch.zeros_like(f64_d)
-    for _ in range(max(1, args.warmup)):
-        engp.apply(f64_d[0], u64_d[0], parts=E.SPME_FULL)
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for k in range(plain_steps):
-        r = k % R
-        engp.apply(f64_d[r], u64_d[r], parts=E.SPME_FULL)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms_plain = ev0.elapsed_time(ev1) / plain_steps
-    for r in range(2):
-        engp.apply(f64_d[r], u64_d[r], parts=E.SPME_FULL)
-    torch.cuda.synchronize()
-    u_plain = u64_d[:2].cpu().numpy()
-    engp.close()
+
+    def fp64_leg(precision):
+        e = E.Engine(xi, K, p, N, L, rc=rc, precision=precision, max_rhs=1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.set_positions(pos_d)
+        torch.cuda.synchronize()
+        pb = time.perf_counter() - t0
+        for _ in range(max(1, args.warmup)):
+            e.apply(f64_d[0], u64_d[0], parts=E.SPME_FULL)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for k in range(plain_steps):
+            r = k % R
+            e.apply(f64_d[r], u64_d[r], parts=E.SPME_FULL)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms = ev0.elapsed_time(ev1) / plain_steps
+        for r in range(2):
+            e.apply(f64_d[r], u64_d[r], parts=E.SPME_FULL)
+        torch.cuda.synchronize()
+        u = u64_d[:2].cpu().numpy()
+        e.close()
+        return ms, pb, u
+
+    ms_plain, pb_plain, u_plain = fp64_leg(E.SPME_FP64_PLAIN)
     plain_value = 1e3 / ms_plain
-    print(f"[bench] fp64 plain: {ms_plain:.3f} ms per M.f, {plain_value:.1f} M.f/s",
+    print(f"[bench] fp64 plain (reference structures): {ms_plain:.3f} ms per M.f, "
+          f"{plain_value:.1f} M.f/s", file=sys.stderr, flush=True)
+    ms_compact, pb_compact, u_compact = fp64_leg(E.SPME_FP64)
+    compact_value = 1e3 / ms_compact
+    print(f"[bench] fp64 compact: {ms_compact:.3f} ms per M.f, {compact_value:.1f} M.f/s",
           file=sys.stderr, flush=True)
 
     # ---- error against the FP64 direct Ewald sum (committed fixture) -------
@@ -308,6 +349,7 @@ def main():
             return float(np.linalg.norm(mine - direct[r]) / np.linalg.norm(direct[r]))
         errors = {"fp32_batched": max(rel(u_full, r) for r in range(direct.shape[0])),
                   "fp64_plain": max(rel(u_plain, r) for r in range(direct.shape[0])),
+                  "fp64_compact": max(rel(u_compact, r) for r in range(direct.shape[0])),
                   "rows": int(rows.size), "rhs_checked": int(direct.shape[0])}
 
     # ---- roofline of the dominant kernel -------------------------------------
@@ -332,6 +374,10 @@ def main():
         multi = multi_system(max(3, args.steps // 4), 2)
         print(f"[bench] multi-system: separate {multi['separate_ms']:.3f} ms, graph "
               f"{multi['graph_ms']:.3f} ms", file=sys.stderr, flush=True)
+
+    copy_ceiling = hbm_copy_ceiling()
+    print(f"[bench] HBM copy ceiling: {copy_ceiling['GBs']:.0f} GB/s", file=sys.stderr,
+          flush=True)
 
     # ---- CPU baseline: restated reference on the host cores ----------------
     cpu = None
@@ -377,11 +423,19 @@ def main():
         "fft_roofline": {"achieved": round(fft_gbs, 1),
                          "frac": round(fft_gbs / HBM_PEAK_GBS, 4),
                          "ms": round(fft_ms, 4)},
+        "hbm_copy_ceiling": copy_ceiling,
         "kernels": per,
         "tiles": tiles,
         "plain_fp64": {"value": round(plain_value, 2), "unit": "M.f/s",
+                       "structures": "reference: dense p^3 P + int32 offsets, atomic "
+                                     "spreading, separate influence and scale passes",
                        "ms_per_mf": round(ms_plain, 4),
+                       "p_build_ms": round(pb_plain * 1e3, 2),
                        "speedup_of_batched_fp32": round(value / plain_value, 2)},
+        "compact_fp64": {"value": round(compact_value, 2), "unit": "M.f/s",
+                         "ms_per_mf": round(ms_compact, 4),
+                         "p_build_ms": round(pb_compact * 1e3, 2),
+                         "speedup_of_batched_fp32": round(value / compact_value, 2)},
         "p_build_ms": round(t_pbuild * 1e3, 2),
         "rel_error_vs_fp64_direct_ewald": errors,
         "multi_system": multi,

@@ -163,6 +163,24 @@ int spme_mlp_backward(const float *w, const float *x, const float *y, const floa
 int spme_mlp_adam(float *w, const float *g, float *m1, float *m2, float lr, float beta1,
                   float beta2, float eps, int step, void *stream);
 
+/* Tall-skinny block operations for block Krylov methods over batches of
+ * right-hand sides (block Lanczos for M^{1/2} z in Brownian dynamics).  No
+ * counterpart in the reference.  Blocks are row-major [s][ld] device arrays
+ * of the precision's type (float for SPME_FP32, double otherwise) with
+ * 1 <= s <= 16 and ld >= n; small matrices are FP64 row-major device arrays.
+ *   gram:   C[a][b] = sum_{i<n} X[a][i] Y[b][i]   (a < sx, b < sy; FP64
+ *           accumulation, deterministic; work: spme_block_gram_worksize
+ *           doubles of device memory)
+ *   update: Y[b][i] = beta Y[b][i] + sum_a X[a][i] A[a][b]  (beta == 0 does
+ *           not read Y; X and Y must not overlap)
+ * Asynchronous on stream (a hipStream_t, NULL: the default stream). */
+long spme_block_gram_worksize(long n, int sx, int sy);
+int spme_block_gram(int precision, long n, int sx, int sy, const void *X, long ldx,
+                    const void *Y, long ldy, double *C, double *work, long work_len,
+                    void *stream);
+int spme_block_update(int precision, long n, int sx, int sy, const void *X, long ldx,
+                      const double *A, double beta, void *Y, long ldy, void *stream);
+
 /* Layout and sizes, for tests and benchmarks. */
 typedef struct spme_info {
   int K, P, N, precision, max_rhs;

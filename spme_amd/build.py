@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspme_hip.so")
 SOURCES = ["spme_group.hip", "spme_kernels.hip", "spme_real.hip", "spme_engine.hip",
-           "spme_mlp.hip", "spme_dense.hip"]
+           "spme_mlp.hip", "spme_dense.hip", "spme_block.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPME_OFFLOAD_ARCH", "gfx950")
 
@@ -36,8 +36,25 @@ def _compile(src: str) -> str:
     return obj
 
 
+PME = os.path.join(HERE, "pme")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+
+def build_cli() -> str:
+    """The pme driver (cli/pme.c), C against include/spme.h and the library."""
+    tmp = PME + ".tmp"
+    cmd = ["gcc", "-O2", "-std=gnu11", "-Wall", f"-I{INCLUDE}", os.path.join(HERE, "cli", "pme.c"),
+           f"-L{HERE}", "-lspme_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib",
+           "-lm", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"pme driver failed to build:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, PME)
+    return PME
+
+
 def build(verbose: bool = True) -> str:
-    """Compile every HIP source and link the shared library."""
+    """Compile every HIP source, link the shared library, build the driver."""
     with ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
@@ -46,8 +63,9 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cli()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {PME}")
     return LIB
 
 

@@ -672,8 +672,7 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
   constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
   constexpr int Kh = K / 2 + 1;
   constexpr int LS = K + 1;
-  using Z = ZLayout<T>;
-  static_assert(Z::XW == XW, "z pass width");
+  using Z = ZLayout<XW>;
   static_assert(XW * N1 <= NT, "one step-B column per thread");
   constexpr int NXC = Z::chunks(K), PER = Z::per_rhs(K);
   load_tw<T, K>(s_tw, tw);

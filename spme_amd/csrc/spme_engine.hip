@@ -40,6 +40,17 @@ int fail(int code, const char *fmt, ...) {
   return code;
 }
 
+}  // namespace
+
+// The error message of the calling thread, for the other translation units
+// of the library (spme_last_error() returns it).
+int spme::set_last_error(int code, const char *msg) {
+  g_err = msg;
+  return code;
+}
+
+namespace {
+
 #define HIP_TRY(expr)                                                                    \
   do {                                                                                   \
     hipError_t e_ = (expr);                                                              \

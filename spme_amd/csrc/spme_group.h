@@ -53,7 +53,7 @@ template <typename T> inline int fft_pass_blocks(int K, int ncomp, int pass) {
     return int((npairs + S::NLX - 1) / S::NLX);
   }
   if (pass == 1 || pass == 3) return ((K / 2 + 1 + S::XW - 1) / S::XW) * K * ncomp;
-  return ZLayout<T>::per_rhs(K) * (ncomp / 3);
+  return ZLayout<S::XWZ>::per_rhs(K) * (ncomp / 3);
 }
 // Largest mesh the grouped FFT kernels are instantiated for: 96 or 256.
 inline int group_kmax(int K) { return K <= 96 ? 96 : 256; }

@@ -297,7 +297,8 @@ int launch_group_real(const GroupStage<RealArgs<T>> &t, int C, int grid, hipStre
       hipLaunchKernelGGL((kg_real<T, CC, false>), dim3(t.total), dim3(GROUP_ELEMS), 0, st, \
                          t);                                                             \
     return 0;
-    X(3) X(6) X(12) X(24) X(48)
+    X(3) X(6) X(9) X(12) X(15) X(18) X(21) X(24) X(27) X(30) X(33) X(36) X(39) X(42) X(45)
+    X(48)
 #undef X
   }
   return -1;

@@ -687,7 +687,8 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
   const long npairs = rows / 2;
   const int Khp = a.RS / 2;
   const int nxc = (K / 2 + 1 + S::XW - 1) / S::XW;
-  const int nzb = ZLayout<T>::per_rhs(K) * (a.ncomp / 3);
+  using ZS = ZShape<T, K>;
+  const int nzb = ZLayout<ZS::XW>::per_rhs(K) * (a.ncomp / 3);
   auto mark = [&](int pass) {
     if (mk) {
       (void)hipEventRecord(mk->ev[mk->n], st);
@@ -702,8 +703,8 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
     hipLaunchKernelGGL((k_fft_xy_fwd<T, K, NTXY>), dim3(grid), dim3(NTXY), 0, st, a.grid, a.RS,
                        a.comp_stride, nplanes, a.tw);
     mark(0);
-    hipLaunchKernelGGL((k_fft_z_infl<T, K, S::XWZ, NT>), dim3(nzb), dim3(NT), 0, st, a.grid,
-                       a.comp_stride, Khp, a.tab, a.dk, a.tw);
+    hipLaunchKernelGGL((k_fft_z_infl<T, K, ZS::XW, ZS::NT>), dim3(nzb), dim3(ZS::NT), 0, st,
+                       a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw);
     mark(2);
     hipLaunchKernelGGL((k_fft_xy_inv<T, K, NTXY>), dim3(grid), dim3(NTXY), 0, st, a.grid, a.RS,
                        a.comp_stride, nplanes, a.tw);
@@ -715,8 +716,8 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
     hipLaunchKernelGGL((k_fft_y<T, K, S::XW, -1, NT>), dim3(nxc * K * a.ncomp), dim3(NT), 0,
                        st, a.grid, a.comp_stride, Khp, nxc, a.tw);
     mark(1);
-    hipLaunchKernelGGL((k_fft_z_infl<T, K, S::XWZ, NT>), dim3(nzb), dim3(NT), 0, st, a.grid,
-                       a.comp_stride, Khp, a.tab, a.dk, a.tw);
+    hipLaunchKernelGGL((k_fft_z_infl<T, K, ZS::XW, ZS::NT>), dim3(nzb), dim3(ZS::NT), 0, st,
+                       a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw);
     mark(2);
     hipLaunchKernelGGL((k_fft_y<T, K, S::XW, +1, NT>), dim3(nxc * K * a.ncomp), dim3(NT), 0,
                        st, a.grid, a.comp_stride, Khp, nxc, a.tw);

@@ -6,6 +6,9 @@ namespace spme {
 
 template <typename T> struct cpx;
 
+// Sets the calling thread's message for spme_last_error(); returns code.
+int set_last_error(int code, const char *msg);
+
 struct WTable {
   double w[64];  // W[x*P + y], P <= 8
 };
@@ -57,12 +60,11 @@ template <typename T> struct FftShape {
   static constexpr int XWZ = sizeof(T) == 8 ? 4 : 8;   // z-pass columns (x3 comps)
 };
 
-// Blocks of the z pass for one right-hand side (3 components): chunks of XWZ
-// x columns for every y.  When K/2 is a multiple of XWZ the chunks cover
-// x < K/2 and the Nyquist column x = K/2 gets K/XWZ narrow blocks of XWZ
-// rows of y each, instead of a chunk that is one column wide.
-template <typename T> struct ZLayout {
-  static constexpr int XW = FftShape<T>::XWZ;
+// Blocks of the z pass for one right-hand side (3 components): chunks of XW
+// x columns for every y.  When K/2 is a multiple of XW the chunks cover
+// x < K/2 and the Nyquist column x = K/2 gets K/XW narrow blocks of XW rows
+// of y each, instead of a chunk that is one column wide.
+template <int XW> struct ZLayout {
   __host__ __device__ static constexpr bool nyq(int K) { return (K / 2) % XW == 0; }
   __host__ __device__ static constexpr int chunks(int K) {
     return nyq(K) ? K / 2 / XW : (K / 2 + XW) / XW;
@@ -70,6 +72,15 @@ template <typename T> struct ZLayout {
   __host__ __device__ static constexpr int per_rhs(int K) {
     return chunks(K) * K + (nyq(K) ? K / XW : 0);
   }
+};
+
+// Single-system z pass shape: FP32 meshes of 128 and 192 points use chunks
+// of whole 128-byte lines (16 complex) with 512 threads; the rest (and the
+// grouped kernels) use FftShape::XWZ columns with 256 threads.
+template <typename T, int K> struct ZShape {
+  static constexpr bool wide = sizeof(T) == 4 && (K == 128 || K == 192);
+  static constexpr int XW = wide ? 16 : FftShape<T>::XWZ;
+  static constexpr int NT = wide ? 512 : 256;
 };
 
 template <typename T> struct FftArgs {

@@ -248,12 +248,12 @@ static void real_c(const RealArgs<T> &a, hipStream_t st) {
 
 template <typename T>
 int launch_real(const RealArgs<T> &a, hipStream_t st) {
-  switch (a.C) {
-    case 3: real_c<T, 3>(a, st); return 0;
-    case 6: real_c<T, 6>(a, st); return 0;
-    case 12: real_c<T, 12>(a, st); return 0;
-    case 24: real_c<T, 24>(a, st); return 0;
-    case 48: real_c<T, 48>(a, st); return 0;
+  switch (a.C) {  // every batch width 1..16
+#define X(CC) \
+  case CC: real_c<T, CC>(a, st); return 0;
+    X(3) X(6) X(9) X(12) X(15) X(18) X(21) X(24) X(27) X(30) X(33) X(36) X(39) X(42) X(45)
+    X(48)
+#undef X
   }
   return -1;
 }

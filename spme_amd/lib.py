@@ -25,6 +25,7 @@ EXPORTED = [
     "spme_engine_set_tiles", "spme_engine_get_tiles", "spme_engine_autotune",
     "spme_group_create", "spme_group_destroy", "spme_group_set_tiles",
     "spme_group_get_tiles", "spme_group_autotune", "spme_group_apply",
+    "spme_block_gram_worksize", "spme_block_gram", "spme_block_update",
 ]
 
 TILE_FIELDS = ["spread_cc", "spread_zc", "interp_zc", "spread_persist",
@@ -111,6 +112,10 @@ def load() -> ctypes.CDLL:
     L.spme_group_autotune.argtypes = [vp, i, i, tp, vp]
     L.spme_group_apply.argtypes = [vp, d, ctypes.POINTER(vp), ctypes.POINTER(l), d,
                                    ctypes.POINTER(vp), ctypes.POINTER(l), i, i, vp]
+    L.spme_block_gram_worksize.argtypes = [l, i, i]
+    L.spme_block_gram_worksize.restype = l
+    L.spme_block_gram.argtypes = [i, l, i, i, vp, l, vp, l, vp, vp, l, vp]
+    L.spme_block_update.argtypes = [i, l, i, i, vp, l, vp, d, vp, l, vp]
     _lib = L
     return L
 
