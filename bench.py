@@ -44,7 +44,7 @@ ROCPROF_NAME = {
     "permute": "k_permute_in<float>",
     "spread": "k_spread<float, 8, 12, 8, 8>",
     "fft_x_fwd": "k_fft_xy_fwd<float, 192, 768>",
-    "fft_z_infl": "k_fft_z_infl<float, 192, 8, 256>",
+    "fft_z_infl": "k_fft_z_infl<float, 192, 16, 512>",
     "fft_x_inv": "k_fft_xy_inv<float, 192, 768>",
     "interp": "k_interp<float, 8, 4, 16, 1024>",
     "permute_real": "k_permute_in<float>",
@@ -63,6 +63,8 @@ def parse():
                     help="skip the CPU baseline leg")
     ap.add_argument("--no-multi", action="store_true",
                     help="skip the uneven multi-system leg")
+    ap.add_argument("--no-brownian", action="store_true",
+                    help="skip the block-Lanczos Brownian displacement leg")
     ap.add_argument("--cpu-rhs", type=int, default=1)
     ap.add_argument("--pmc-run", action="store_true",
                     help="short run for hardware-counter passes: a plain copy of "
@@ -150,6 +152,32 @@ def traffic_per_launch(kernel):
         if m and m.group(1) == kernel:
             return float(m.group(2)) * 1e6  # the table gives MB
     return None
+
+
+def brownian_leg(eng, F, R, stream, dt=1e-3, kT=1.0, tol=1e-4):
+    """dX = dt M F + sqrt(2 kT dt) M^{1/2} Z for R realisations at the bench's
+    workload: block Lanczos on the batched FP32 apply (tolerance 1e-4)."""
+    import torch
+    from spme_amd import workload as W
+    from spme_amd.lanczos import brownian_displacements
+    N = F.shape[1] // 3
+    Z = torch.tensor(W.forces(2024, N, R), device=F.device, dtype=F.dtype)
+    brownian_displacements(eng, F, Z, dt, kT, tol=tol)   # warm-up
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    dX, info = brownian_displacements(eng, F, Z, dt, kT, tol=tol)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3
+    applies = info.iterations + 1
+    return {"what": f"{R} displacements dX = dt M F + sqrt(2 kT dt) M^1/2 Z, block Lanczos "
+                    f"(full reorthogonalisation) on the batched FP32 apply, tol {tol}",
+            "iterations": info.iterations, "krylov_dimension": info.dimension,
+            "change": info.change, "ms": round(wall, 2),
+            "ms_gpu_span": round(e0.elapsed_time(e1), 2), "batched_applies": applies,
+            "ritz_min": info.min_eig, "ritz_max": info.max_eig}
 
 
 def hbm_copy_ceiling(gib=2, reps=10):
@@ -297,6 +325,13 @@ def main():
     print(f"[bench] fp32 batched: {ms_step:.3f} ms/step, {value:.1f} M.f/s "
           f"(wall {wall:.3f}s)", file=sys.stderr, flush=True)
     u_full = u32_d.double().cpu().numpy()
+
+    # ---- Brownian displacements: block Lanczos over the batched apply -------
+    brownian = None
+    if not args.no_brownian:
+        brownian = brownian_leg(eng, f32_d, R, stream)
+        print(f"[bench] brownian: {brownian['iterations']} Lanczos steps, "
+              f"{brownian['ms']:.1f} ms per block of {R}", file=sys.stderr, flush=True)
     eng.close()
 
     # ---- FP64 paths, one right-hand side per apply ---------------------------
@@ -439,6 +474,7 @@ def main():
         "p_build_ms": round(t_pbuild * 1e3, 2),
         "rel_error_vs_fp64_direct_ewald": errors,
         "multi_system": multi,
+        "brownian": brownian,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

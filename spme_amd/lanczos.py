@@ -140,6 +140,20 @@ class BlockLanczos:
         ops.update(V1, np.linalg.inv(R2), 0.0, V)
         return V, R2 @ R
 
+    @staticmethod
+    def _coefficients(As, Bs, R0):
+        """(T^{1/2})[:, :k0] R0 for the block tridiagonal T of As, Bs."""
+        sizes = [a.shape[0] for a in As]
+        off = np.concatenate([[0], np.cumsum(sizes)])
+        T = np.zeros((off[-1], off[-1]))
+        for i, a in enumerate(As):
+            T[off[i]:off[i + 1], off[i]:off[i + 1]] = a
+            if i + 1 < len(As):
+                T[off[i + 1]:off[i + 2], off[i]:off[i + 1]] = Bs[i]
+                T[off[i]:off[i + 1], off[i + 1]:off[i + 2]] = Bs[i].T
+        S, lo, hi = _sqrt_first_cols(T, R0.shape[0])
+        return S @ R0, lo, hi, off
+
     def sqrt(self, Z, out=None):
         """Return (U, info) with U ~ M^{1/2} Z (rows of Z are the vectors)."""
         s, ops = self.s, self.ops
@@ -153,6 +167,8 @@ class BlockLanczos:
         Qs, As, Bs = [Q], [], []
         scale, coef_prev, change = 0.0, None, np.inf
         invariant = False
+        next_check = 1
+        coef, lo, hi, checked = None, 0.0, 0.0, 0
         for j in range(self.max_iter):
             kj = Qs[j].shape[0]
             W = ops.empty(kj)
@@ -167,29 +183,29 @@ class BlockLanczos:
             if self.reorth:
                 for Qi in Qs:
                     ops.update(Qi, -ops.gram(Qi, W), 1.0, W)
-            sizes = [a.shape[0] for a in As]
-            off = np.concatenate([[0], np.cumsum(sizes)])
-            T = np.zeros((off[-1], off[-1]))
-            for i, a in enumerate(As):
-                T[off[i]:off[i + 1], off[i]:off[i + 1]] = a
-                if i + 1 < len(As):
-                    T[off[i + 1]:off[i + 2], off[i]:off[i + 1]] = Bs[i]
-                    T[off[i]:off[i + 1], off[i + 1]:off[i + 2]] = Bs[i].T
-            S, lo, hi = _sqrt_first_cols(T, R0.shape[0])
-            coef = S @ R0
-            if coef_prev is not None:
-                pad = np.zeros_like(coef)
-                pad[:coef_prev.shape[0]] = coef_prev
-                change = float(np.linalg.norm(coef - pad) / np.linalg.norm(coef))
-            coef_prev = coef
-            if change < self.tol or j + 1 == self.max_iter:
-                break
+            m = j + 1
+            # the (s m)^3 eigendecomposition is checked every step while it is
+            # cheap, then every m/4 steps (the change is then measured over
+            # several steps, a stricter test)
+            if m >= next_check or m == self.max_iter:
+                coef, lo, hi, off = self._coefficients(As, Bs, R0)
+                checked = m
+                if coef_prev is not None:
+                    pad = np.zeros_like(coef)
+                    pad[:coef_prev.shape[0]] = coef_prev
+                    change = float(np.linalg.norm(coef - pad) / np.linalg.norm(coef))
+                coef_prev = coef
+                next_check = m + (1 if m < 8 else max(1, m // 4))
+                if change < self.tol or m == self.max_iter:
+                    break
             Qn, B = self._orth(W, scale)
             if Qn is None:       # invariant space: the result is exact
                 invariant = True
                 break
             Qs.append(Qn)
             Bs.append(B)
+        if checked != len(As):
+            coef, lo, hi, off = self._coefficients(As, Bs, R0)
         m = len(As)
         info = LanczosInfo(m, change, invariant or change < self.tol, lo, hi, int(off[-1]),
                            invariant)
@@ -198,7 +214,7 @@ class BlockLanczos:
         return U, info
 
 
-def engine_lanczos(engine, s: int, tol: float = 1e-6, max_iter: int = 60,
+def engine_lanczos(engine, s: int, tol: float = 1e-6, max_iter: int = 100,
                    parts: int = SPME_FULL) -> BlockLanczos:
     """Block Lanczos over an engine's batched apply (full operator)."""
     if s > engine.max_rhs:
@@ -213,7 +229,7 @@ def engine_lanczos(engine, s: int, tol: float = 1e-6, max_iter: int = 60,
 
 
 def brownian_displacements(engine, F, Z, dt: float, kT: float, tol: float = 1e-4,
-                           max_iter: int = 60):
+                           max_iter: int = 100):
     """dX = dt M F + sqrt(2 kT dt) M^{1/2} Z for s independent realisations
     (rows of F and Z, [s][3N] device tensors of the engine's precision;
     mobility in units of 1/(6 pi eta a)).  Returns (dX, info)."""
