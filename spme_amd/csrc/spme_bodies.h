@@ -253,13 +253,22 @@ __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
   constexpr int S = InterpPhase<P>::S, MAXB = InterpPhase<P>::MAXB;
   constexpr int NSLOT = InterpPhase<P>::NSLOT;
   static_assert((NSLOT & (NSLOT - 1)) == 0 && S + P - 1 == NSLOT, "ring of planes");
-  constexpr int E = TXY + P - 1;
-  constexpr int PLANE = E * E;          // points per plane
+  constexpr int E = TXY + P - 1;        // tile + halo points per side
   constexpr int NSUB = TXY / 8;         // sort tiles are 8 x 8
   constexpr int NS2 = NSUB * NSUB;
   constexpr int NWV = NT / 64;
   constexpr int NW = 3 * P;
-  constexpr int NPT = (S * PLANE + NT - 1) / NT;   // plane points per thread per phase
+  // Window row pitch, in 16-byte points: a wave's 16-lane phase of a b128
+  // read covers two stencil rows, which fall in disjoint halves of the 64
+  // banks when the pitch is 8 mod 16.  Padded where it still fits in LDS.
+  constexpr int EP_PAD = E + ((8 - E) % 16 + 16) % 16;
+  constexpr bool PAD_FITS =
+      (size_t)NSLOT * E * EP_PAD * 16 + (size_t)MAXB * NW * sizeof(T) + 2 * MAXB * 4 <=
+      160 * 1024;
+  constexpr int EP = PAD_FITS ? EP_PAD : E;
+  constexpr int NPTS = E * E;           // points of a plane
+  constexpr int PLANE = E * EP;         // window slot stride
+  constexpr int NPT = (S * NPTS + NT - 1) / NT;    // plane points per thread per phase
   constexpr int RW = (MAXB * NW + NT - 1) / NT;    // weights per thread per batch
   __shared__ V win[NSLOT * PLANE];
   __shared__ int s_pb[MAXB], s_perm[MAXB];
@@ -283,13 +292,18 @@ __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
     key0[q] = (sy * (K / 8) + sx) * K;
   }
 
-  // plane point p of the tile (p < PLANE): source offset in a mesh plane
+  // plane point p of the tile (p < NPTS): source offset in a mesh plane
+  // and position in a window slot
   auto src_of = [&](int p) {
     const int ey = p / E, ex = p - ey * E;
     int yy = y0 + ey, xx = x0 + ex;
     yy -= yy >= K ? K : 0;
     xx -= xx >= K ? K : 0;
     return (long)yy * RS + xx;
+  };
+  auto win_of = [&](int p) {
+    const int ey = p / E;
+    return ey * EP + (p - ey * E);
   };
   auto load_point = [&](int z, int p) {  // z unwrapped
     V v;
@@ -299,9 +313,9 @@ __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
     return v;
   };
   // prologue: planes z0 .. z0+P-2
-  for (int q = tid; q < (P - 1) * PLANE; q += NT) {
-    const int j = q / PLANE, p = q - j * PLANE;
-    win[((z0 + j) & (NSLOT - 1)) * PLANE + p] = load_point(z0 + j, p);
+  for (int q = tid; q < (P - 1) * NPTS; q += NT) {
+    const int j = q / NPTS, p = q - j * NPTS;
+    win[((z0 + j) & (NSLOT - 1)) * PLANE + win_of(p)] = load_point(z0 + j, p);
   }
   // the phase loads of this thread: plane j_of[r] of the phase, point with
   // in-plane offset src_off[r] (mesh) and pt_of[r] (window); fixed over the
@@ -311,9 +325,9 @@ __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
 #pragma unroll
   for (int r = 0; r < NPT; ++r) {
     const int q = tid + NT * r;
-    const int j = q / PLANE, p = q - j * PLANE;
-    j_of[r] = q < S * PLANE ? j : S;  // S: no load
-    pt_of[r] = p;
+    const int j = q / NPTS, p = q - j * NPTS;
+    j_of[r] = q < S * NPTS ? j : S;  // S: no load
+    pt_of[r] = win_of(p);
     src_off[r] = src_of(p);
   }
 
@@ -402,7 +416,7 @@ __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
 #pragma unroll
       for (int c = 0; c < CC; ++c) acc[c] = T(0);
       const int sl0 = bz & (NSLOT - 1);
-      const V *pt = win + (ry + dyc) * E + rx + dxc;
+      const V *pt = win + (ry + dyc) * EP + rx + dxc;
       if (sl0 + P <= NSLOT) {  // the particle's planes sit in consecutive slots
         const V *pz = pt + sl0 * PLANE;
 #pragma unroll

@@ -450,11 +450,17 @@ __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS,
     __syncthreads();
     // y inverse, step B, natural order: in rounds of whole columns (a
     // column's tasks read and write the same positions), one register tile
-    // per thread
+    // per thread.  When the columns before the Nyquist column fill whole
+    // rounds, the Nyquist column x = K/2 is done in the last round by the
+    // first N1 lanes of wave 0 after their own column: its 16 tasks lie in
+    // one wave, and each task's writes wait on its reads through the FFT,
+    // so it needs no barrier of its own (a round of its own would cost two).
     constexpr int CPR = NT / N1;  // columns per round
-    for (int x0 = 0; x0 < Kh; x0 += CPR) {
+    constexpr bool NYQ_IN_ROUND = (Kh - 1) % CPR == 0 && N1 <= 64;
+    constexpr int KX = NYQ_IN_ROUND ? Kh - 1 : Kh;  // columns done in rounds
+    for (int x0 = 0; x0 < KX; x0 += CPR) {
       const int x = x0 + tid % CPR, k1 = tid / CPR;  // consecutive lanes: consecutive x
-      const bool on = tid < CPR * N1 && x < Kh;
+      const bool on = tid < CPR * N1 && x < KX;
       cpx<T> c[N2];
       if (on) {
 #pragma unroll
@@ -465,6 +471,14 @@ __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS,
       if (on) {
 #pragma unroll
         for (int k2 = 0; k2 < N2; ++k2) buf[(k1 + N1 * k2) * Kh + x] = c[k2];
+      }
+      if (NYQ_IN_ROUND && x0 + CPR >= KX && tid < N1) {
+        const int kn = tid;  // task k1 of column K/2
+#pragma unroll
+        for (int n2 = 0; n2 < N2; ++n2) c[n2] = buf[(kn * N2 + n2) * Kh + K / 2];
+        reg_fft<N2, +1>(c);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) buf[(kn + N1 * k2) * Kh + K / 2] = c[k2];
       }
       __syncthreads();
     }
