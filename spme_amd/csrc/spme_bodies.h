@@ -781,14 +781,39 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
   }
 }
 
-// Real-space SpMV, one lane per (row, right-hand side): the R = C/3 lanes
-// of a row (cell order) read a neighbour's forces as one contiguous segment
-// and share the pair's block and column index.  fs: forces in cell order
-// [N][C].  For right-hand side r and d < 3:
+// Real-space SpMV: a row (cell order) is handled by LPR lanes, each with RPL
+// consecutive right-hand sides, so the pair's block and column index are
+// loaded once per RPL right-hand sides and the neighbour's 3 RPL forces come
+// as 16-byte loads.  fs: forces in cell order [N][C].  For right-hand side r
+// and d < 3:
 //   out[r*ldout + 3*perm[s] + d] = (accum ? out : beta*out)
 //                                  + alpha*(self f + sum_j B_sj f_j).
 // Rows are packed in slices of 64 (entry-major within a slice); the rows of
 // a wave lie in one slice, so every lane runs the same entry count.
+template <int C> struct RealShape {
+  static constexpr int RPL = real_rpl(C);  // right-hand sides per lane
+  static constexpr int LPR = real_lpr(C);  // lanes per row
+};
+
+// N values from p into v; 16-byte loads when N values fill whole 16-byte
+// units (p is then 16-byte aligned at every call site).
+template <typename T, int N>
+__device__ __forceinline__ void load_n(const T *__restrict__ p, T (&v)[N]) {
+  if constexpr (N * sizeof(T) % 16 == 0) {
+    constexpr int W = 16 / sizeof(T);
+    struct alignas(16) U { T v[W]; };
+#pragma unroll
+    for (int i = 0; i < N / W; ++i) {
+      const U t = reinterpret_cast<const U *>(p)[i];
+#pragma unroll
+      for (int k = 0; k < W; ++k) v[W * i + k] = t.v[k];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = p[i];
+  }
+}
+
 template <typename T, int C>
 __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
                                           const int *__restrict__ width,
@@ -796,30 +821,40 @@ __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
                                           const int *__restrict__ col,
                                           const T *__restrict__ blk, long E, T self,
                                           const T *__restrict__ fs, T alpha, T beta, int accum,
-                                          T *__restrict__ out, long ldout, int s, int r) {
-  const int q = s >> 6, l = s & 63;
-  const T *fr = fs + (size_t)s * C + 3 * r;
-  T a0 = self * fr[0], a1 = self * fr[1], a2 = self * fr[2];
-  const long base = soff[q] + l;
+                                          T *__restrict__ out, long ldout, int s, int l) {
+  constexpr int RPL = RealShape<C>::RPL, NV = 3 * RPL;
+  const int q = s >> 6, ls = s & 63;
+  const int r0 = l * RPL;
+  T a[NV], f[NV];
+  load_n<T, NV>(fs + (size_t)s * C + 3 * r0, f);
+#pragma unroll
+  for (int i = 0; i < NV; ++i) a[i] = self * f[i];
+  const long base = soff[q] + ls;
   const int w = width[q];
-#pragma unroll 8
+#pragma unroll 4
   for (int k = 0; k < w; ++k) {
     const long e = base + 64L * k;
     const int j = col[e];
     const T b0 = blk[e], b1 = blk[E + e], b2 = blk[2 * E + e];
     const T b3 = blk[3 * E + e], b4 = blk[4 * E + e], b5 = blk[5 * E + e];
-    const T *fj = fs + (size_t)j * C + 3 * r;
-    const T f0 = fj[0], f1 = fj[1], f2 = fj[2];
-    a0 += b0 * f0 + b1 * f1 + b2 * f2;
-    a1 += b1 * f0 + b3 * f1 + b4 * f2;
-    a2 += b2 * f0 + b4 * f1 + b5 * f2;
-  }
-  T *u = out + r * ldout + 3L * perm[s];
-  const T acc[3] = {a0, a1, a2};
+    load_n<T, NV>(fs + (size_t)j * C + 3 * r0, f);
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const T prev = accum ? u[d] : (beta == T(0) ? T(0) : beta * u[d]);
-    u[d] = prev + alpha * acc[d];
+    for (int r = 0; r < RPL; ++r) {
+      const T f0 = f[3 * r], f1 = f[3 * r + 1], f2 = f[3 * r + 2];
+      a[3 * r] += b0 * f0 + b1 * f1 + b2 * f2;
+      a[3 * r + 1] += b1 * f0 + b3 * f1 + b4 * f2;
+      a[3 * r + 2] += b2 * f0 + b4 * f1 + b5 * f2;
+    }
+  }
+  const long po = 3L * perm[s];
+#pragma unroll
+  for (int r = 0; r < RPL; ++r) {
+    T *u = out + (r0 + r) * ldout + po;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T prev = accum ? u[d] : (beta == T(0) ? T(0) : beta * u[d]);
+      u[d] = prev + alpha * a[3 * r + d];
+    }
   }
 }
 

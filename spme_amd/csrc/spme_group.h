@@ -40,9 +40,9 @@ constexpr int GROUP_ELEMS = 256;  // elements (rows) per logical block
 inline int group_perm_blocks(int N, int C) {
   return int(((long)N * C + GROUP_ELEMS - 1) / GROUP_ELEMS);
 }
-// real space: one lane per (row, right-hand side)
+// real space: real_lpr(C) lanes per row
 inline int group_real_blocks(int N, int C) {
-  return int(((long)N * (C / 3) + GROUP_ELEMS - 1) / GROUP_ELEMS);
+  return int(((long)N * real_lpr(C) + GROUP_ELEMS - 1) / GROUP_ELEMS);
 }
 // Logical blocks of FFT pass i (0 x fwd, 1 y fwd, 2 z + influence,
 // 3 y inv, 4 x inv) of a K^3 mesh with ncomp components.

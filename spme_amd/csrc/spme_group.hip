@@ -80,12 +80,12 @@ __global__ __launch_bounds__(GROUP_ELEMS) void kg_real(GroupStage<RealArgs<T>> t
     int lb;
     const int s = group_locate(t, g, true, lb);
     const RealArgs<T> &a = t.args[s];
-    constexpr int R = C / 3;
+    constexpr int LPR = RealShape<C>::LPR;
     const long gi = (long)lb * GROUP_ELEMS + threadIdx.x;
-    const int row = int(gi / R), r = int(gi % R);
+    const int row = int(gi / LPR), l = int(gi % LPR);
     if (row < a.N)
       real_lane<T, C>(a.perm, a.width, a.soff, a.col, a.blk, a.E, T(a.self), a.fs, T(a.alpha),
-                      T(a.beta), a.accum, a.out, a.ldout, row, r);
+                      T(a.beta), a.accum, a.out, a.ldout, row, l);
   });
 }
 

@@ -60,6 +60,13 @@ template <typename T> struct FftShape {
   static constexpr int XWZ = sizeof(T) == 8 ? 4 : 8;   // z-pass columns (x3 comps)
 };
 
+// Real space: lanes per row for C = 3R components, each lane taking 4, 2 or 1
+// consecutive right-hand sides.
+__host__ __device__ constexpr int real_rpl(int C) {
+  return (C / 3) % 4 == 0 ? 4 : ((C / 3) % 2 == 0 ? 2 : 1);
+}
+__host__ __device__ constexpr int real_lpr(int C) { return C / 3 / real_rpl(C); }
+
 // Blocks of the z pass for one right-hand side (3 components): chunks of XW
 // x columns for every y.  When K/2 is a multiple of XW the chunks cover
 // x < K/2 and the Nyquist column x = K/2 gets K/XW narrow blocks of XW rows

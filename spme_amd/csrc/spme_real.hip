@@ -199,12 +199,12 @@ __global__ __launch_bounds__(256) void k_real_spmv(const int *__restrict__ perm,
                                                    T self, const T *__restrict__ fs, int N,
                                                    T alpha, T beta, int accum,
                                                    T *__restrict__ out, long ldout) {
-  constexpr int R = C / 3;
+  constexpr int LPR = RealShape<C>::LPR;
   const long g = (long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const int s = int(g / R), r = int(g % R);
+  const int s = int(g / LPR), l = int(g % LPR);
   if (s < N)
     real_lane<T, C>(perm, width, soff, col, blk, E, self, fs, alpha, beta, accum, out, ldout, s,
-                    r);
+                    l);
 }
 
 static int grid1(long n, int b) { return int((n + b - 1) / b); }
@@ -240,7 +240,8 @@ void launch_pair_fill(const double *spos, const int *coff, int nc, double L, dou
 
 template <typename T, int C>
 static void real_c(const RealArgs<T> &a, hipStream_t st) {
-  hipLaunchKernelGGL((k_real_spmv<T, C>), dim3(grid1((long)a.N * (C / 3), 256)), dim3(256), 0,
+  hipLaunchKernelGGL((k_real_spmv<T, C>), dim3(grid1((long)a.N * RealShape<C>::LPR, 256)),
+                     dim3(256), 0,
                      st, a.perm,
                      a.width, a.soff, a.col, a.blk, a.E, T(a.self), a.fs, a.N, T(a.alpha),
                      T(a.beta), a.accum, a.out, a.ldout);
