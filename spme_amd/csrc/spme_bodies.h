@@ -6,6 +6,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "spme_kernels.h"
 #include "spme_launch.h"
 
@@ -33,6 +35,18 @@ __device__ __forceinline__ void permute_elem(const int *__restrict__ perm,
 // Latency: the bucket offsets of the whole sweep are read into LDS up front,
 // and each step's particles (base, weights, forces) are staged into LDS with
 // one batch of independent loads.
+// f(st0 + U, integral_constant<U>) for U < P while st0 + U < n: a loop of P
+// steps unrolled so that U is a constant in each copy of the body.
+template <int U, int P, class F>
+__device__ __forceinline__ void unroll_steps(int st0, int n, F &f) {
+  if constexpr (U < P) {
+    if (st0 + U < n) {
+      f(st0 + U, std::integral_constant<int, U>{});
+      unroll_steps<U + 1, P>(st0, n, f);
+    }
+  }
+}
+
 template <typename T, int P, int CC, int TX, int TY>
 __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
                                              const int *__restrict__ pbase,
@@ -81,7 +95,13 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
 #pragma unroll
     for (int c = 0; c < CC; ++c) acc[j][c] = T(0);
 
-  for (int st = 0; st < nsteps; ++st) {
+  // Step st adds the particles whose stencil starts at plane zs + st to the
+  // planes zs + st + j, j < P, held in acc[(st + j) % P]; after the step,
+  // plane zs + st (slot st % P) is complete: it is written and its slot
+  // cleared for plane zs + st + P.  The steps are unrolled by P so that
+  // every slot index is a constant (no register moves between steps).
+  auto step = [&](int st, auto uc) {
+    constexpr int u = decltype(uc)::value;  // st % P
     int cnt[NTL], lo[NTL];
     int n = 0;
 #pragma unroll
@@ -145,7 +165,7 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
           for (int j = 0; j < P; ++j) {
             const T wz = w[2 * P + j];
 #pragma unroll
-            for (int c = 0; c < CC; ++c) acc[j][c] += wz * wf[c];
+            for (int c = 0; c < CC; ++c) acc[(u + j) % P][c] += wz * wf[c];
           }
         }
       }
@@ -154,15 +174,12 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
     if (st >= P - 1) {
       T *g = grid + ((size_t)(zs + st) * K + y) * RS + x + (size_t)cc0 * comp_stride;
 #pragma unroll
-      for (int c = 0; c < CC; ++c) g[(size_t)c * comp_stride] = acc[0][c];
+      for (int c = 0; c < CC; ++c) g[(size_t)c * comp_stride] = acc[u][c];
     }
 #pragma unroll
-    for (int j = 0; j < P - 1; ++j)
-#pragma unroll
-      for (int c = 0; c < CC; ++c) acc[j][c] = acc[j + 1][c];
-#pragma unroll
-    for (int c = 0; c < CC; ++c) acc[P - 1][c] = T(0);
-  }
+    for (int c = 0; c < CC; ++c) acc[u][c] = T(0);
+  };
+  for (int st0 = 0; st0 < nsteps; st0 += P) unroll_steps<0, P>(st0, nsteps, step);
 }
 
 // ---------------------------------------------------------------------------
