@@ -63,8 +63,8 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
   __shared__ int s_lo[NTL * MAXST];
   __shared__ int s_n[NTL * MAXST];
   __shared__ int s_pb[MAXB];
-  __shared__ T s_w[MAXB * NW];
-  __shared__ T s_f[MAXB * CC];
+  __shared__ __attribute__((aligned(16))) T s_w[MAXB * NW];
+  __shared__ __attribute__((aligned(16))) T s_f[MAXB * CC];
   const int ntx = K / TX, nty = K / TY, nz = (K + ZC - 1) / ZC;
   const int tile = lb % (ntx * nty), rest = lb / (ntx * nty);
   const int tx = tile % ntx, ty = tile / ntx;
@@ -149,24 +149,36 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
       for (int r = 0; r < RF; ++r)
         if (lane + 64 * r < MAXB * CC) s_f[lane + 64 * r] = vf[r];
       __syncthreads();
+      // the batch's particles whose stencil covers this lane's column; the
+      // wave then runs as many rounds as its busiest lane has particles, each
+      // lane taking its own next particle (ascending, as a plain loop would)
+      unsigned mine = 0;
       for (int k = 0; k < nb; ++k) {
         const int pb = s_pb[k];
         int dx = x - (pb & 1023);
         dx += dx < 0 ? K : 0;
         int dy = y - ((pb >> 10) & 1023);
         dy += dy < 0 ? K : 0;
-        if (dx < P && dy < P) {
-          const T *w = s_w + k * NW;
-          const T wxy = w[dx] * w[P + dy];
-          T wf[CC];
+        mine |= (dx < P && dy < P) ? 1u << k : 0u;
+      }
+      while (mine) {
+        const int k = __ffs(mine) - 1;
+        mine &= mine - 1;
+        const int pb = s_pb[k];
+        int dx = x - (pb & 1023);
+        dx += dx < 0 ? K : 0;
+        int dy = y - ((pb >> 10) & 1023);
+        dy += dy < 0 ? K : 0;
+        const T *w = s_w + k * NW;
+        const T wxy = w[dx] * w[P + dy];
+        T wf[CC];
 #pragma unroll
-          for (int c = 0; c < CC; ++c) wf[c] = wxy * s_f[k * CC + c];
+        for (int c = 0; c < CC; ++c) wf[c] = wxy * s_f[k * CC + c];
 #pragma unroll
-          for (int j = 0; j < P; ++j) {
-            const T wz = w[2 * P + j];
+        for (int j = 0; j < P; ++j) {
+          const T wz = w[2 * P + j];
 #pragma unroll
-            for (int c = 0; c < CC; ++c) acc[(u + j) % P][c] += wz * wf[c];
-          }
+          for (int c = 0; c < CC; ++c) acc[(u + j) % P][c] += wz * wf[c];
         }
       }
       __syncthreads();
