@@ -32,9 +32,18 @@ __device__ __forceinline__ void permute_elem(const int *__restrict__ perm,
 // stored) once the sweep has passed every base that reaches it; the window
 // then shifts by one plane.  Every cell is written exactly once, so the
 // mesh needs no clearing and no atomics are used.
-// Latency: the bucket offsets of the whole sweep are read into LDS up front,
-// and each step's particles (base, weights, forces) are staged into LDS with
-// one batch of independent loads.
+// Latency: the bucket offsets of a sweep (at most 96 planes; longer z
+// chunks run several sweeps, which changes no sum) are read into LDS up
+// front.  The particles (base, weights, forces) of G consecutive steps are
+// one staging group (bucket ranges of consecutive bases are contiguous in
+// the sorted order): they are copied into LDS by direct global->LDS loads
+// (no registers), double-buffered, so the next group's copy is in flight
+// while the current group is spread.  A group that wraps through z = 0 or
+// exceeds a batch is staged step by step instead.  Each step adds its
+// particles in ascending sorted order per tile either way, so the result
+// does not depend on the staging.  The block is one wave: no barriers (LDS
+// accesses of one wave execute in order; the compiler waits for a direct
+// load before the LDS reads it feeds).
 // f(st0 + U, integral_constant<U>) for U < P while st0 + U < n: a loop of P
 // steps unrolled so that U is a constant in each copy of the body.
 template <int U, int P, class F>
@@ -47,6 +56,23 @@ __device__ __forceinline__ void unroll_steps(int st0, int n, F &f) {
   }
 }
 
+// Steps per staging group (an even number of groups per P unrolled steps,
+// so the two buffers alternate) and particles per batch (one bit each of a
+// 64-bit mask; about twice a group's expected count at 0.03-0.05 particles
+// per cell).
+template <int P> struct SpreadStage {
+  static constexpr int G = P % 4 == 0 ? P / 4 : P / 2;
+  static constexpr int MAXB = P % 4 == 0 ? 32 : 64;
+};
+
+// s_waitcnt vmcnt(0): every vector memory access of the wave, direct LDS
+// loads included, has completed.  The empty asm keeps the compiler from
+// moving memory accesses across the wait (the wait itself touches no memory).
+__device__ __forceinline__ void wait_vm_all() {
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  asm volatile("" ::: "memory");
+}
+
 template <typename T, int P, int CC, int TX, int TY>
 __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
                                              const int *__restrict__ pbase,
@@ -57,14 +83,21 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
   constexpr int NXR = 1 + (P + TX - 2) / TX;
   constexpr int NYR = 1 + (P + TY - 2) / TY;
   constexpr int NTL = NXR * NYR;
-  constexpr int MAXST = 256 + P;  // >= ZC + P - 1
-  constexpr int MAXB = 16;        // particles staged per batch
+  constexpr int MAXZ = 96;            // planes per sweep
+  constexpr int MAXST = MAXZ + P - 1;
+  constexpr int G = SpreadStage<P>::G, MAXB = SpreadStage<P>::MAXB;
+  static_assert(P % (2 * G) == 0 && MAXB <= 64, "buffers alternate per group");
+  using Mask = std::conditional_t<(MAXB > 32), unsigned long long, unsigned>;
+  constexpr Mask ONE = 1;
   constexpr int NW = 3 * P;
+  constexpr int Q = sizeof(T) / 4;                 // dwords per value
+  constexpr int RWD = (MAXB * NW * Q + 63) / 64;   // weight dwords per lane
+  constexpr int RFD = (MAXB * CC * Q + 63) / 64;   // force dwords per lane
   __shared__ int s_lo[NTL * MAXST];
   __shared__ int s_n[NTL * MAXST];
-  __shared__ int s_pb[MAXB];
-  __shared__ __attribute__((aligned(16))) T s_w[MAXB * NW];
-  __shared__ __attribute__((aligned(16))) T s_f[MAXB * CC];
+  __shared__ int s_pb0[MAXB], s_pb1[MAXB];
+  __shared__ __attribute__((aligned(16))) T s_w0[MAXB * NW], s_w1[MAXB * NW];
+  __shared__ __attribute__((aligned(16))) T s_f0[MAXB * CC], s_f1[MAXB * CC];
   const int ntx = K / TX, nty = K / TY, nz = (K + ZC - 1) / ZC;
   const int tile = lb % (ntx * nty), rest = lb / (ntx * nty);
   const int tx = tile % ntx, ty = tile / ntx;
@@ -72,126 +105,200 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
   const int x = tx * TX + lane % TX, y = ty * TY + lane / TX;
   const int z0 = (rest % nz) * ZC, zend = min(z0 + ZC, K);
   const int cc0 = (rest / nz) * CC;
-  const int zs = z0 - (P - 1);
-  const int nsteps = zend - zs;
+  const unsigned *pw_d = reinterpret_cast<const unsigned *>(pw);
+  const unsigned *fs_d = reinterpret_cast<const unsigned *>(fs);
 
-  for (int q = lane; q < NTL * nsteps; q += 64) {
-    const int t = q / nsteps, st = q - t * nsteps;
-    int tyy = ty - t / NXR, txx = tx - t % NXR;
-    tyy += tyy < 0 ? nty : 0;
-    txx += txx < 0 ? ntx : 0;
-    int bz = zs + st;
-    bz += bz < 0 ? K : 0;
-    const int key = (tyy * ntx + txx) * K + bz;
-    const int lo = boff[key];
-    s_lo[q] = lo;
-    s_n[q] = boff[key + 1] - lo;
-  }
-  __syncthreads();
-
-  T acc[P][CC];
-#pragma unroll
-  for (int j = 0; j < P; ++j)
-#pragma unroll
-    for (int c = 0; c < CC; ++c) acc[j][c] = T(0);
-
-  // Step st adds the particles whose stencil starts at plane zs + st to the
-  // planes zs + st + j, j < P, held in acc[(st + j) % P]; after the step,
-  // plane zs + st (slot st % P) is complete: it is written and its slot
-  // cleared for plane zs + st + P.  The steps are unrolled by P so that
-  // every slot index is a constant (no register moves between steps).
-  auto step = [&](int st, auto uc) {
-    constexpr int u = decltype(uc)::value;  // st % P
-    int cnt[NTL], lo[NTL];
-    int n = 0;
-#pragma unroll
-    for (int t = 0; t < NTL; ++t) {
-      cnt[t] = s_n[t * nsteps + st];
-      lo[t] = s_lo[t * nsteps + st];
-      n += cnt[t];
-    }
-    // sorted index of the k-th particle of this step
+  // copy particles [b0, b0 + nb) of the concatenated sorted ranges
+  // (lo[t], cnt[t]), t < NTL, into buffer B: base, NW weights and the CC
+  // forces of this pass, one dword per lane per load
+  auto stage = [&](auto bc, const int (&lo)[NTL], const int (&cnt)[NTL], int b0, int nb) {
+    constexpr int B = decltype(bc)::value;
+    int *spb = B ? s_pb1 : s_pb0;
+    T *sw = B ? s_w1 : s_w0;
+    T *sf = B ? s_f1 : s_f0;
     auto locate = [&](int k) {
-      int i = lo[0] + k;
+      int i = lo[0] + k, rem = k - cnt[0];
 #pragma unroll
-      for (int t = 0; t < NTL - 1; ++t) {
-        if (k >= cnt[t]) {
-          k -= cnt[t];
-          i = lo[t + 1] + k;
-        } else {
-          break;
-        }
+      for (int t = 1; t < NTL; ++t) {
+        i = rem >= 0 ? lo[t] + rem : i;
+        rem -= cnt[t];
       }
       return i;
     };
-    for (int b0 = 0; b0 < n; b0 += MAXB) {
-      const int nb = min(MAXB, n - b0);
-      // stage the batch: all loads first, then the LDS stores
-      constexpr int RW = (MAXB * NW + 63) / 64, RF = (MAXB * CC + 63) / 64;
-      int vpb = 0;
-      T vw[RW], vf[RF];
-      if (lane < nb) vpb = pbase[locate(b0 + lane)];
+    if (lane < nb) __builtin_amdgcn_global_load_lds(pbase + locate(b0 + lane), spb, 4, 0, 0);
 #pragma unroll
-      for (int r = 0; r < RW; ++r) {
-        const int g = lane + 64 * r;
-        vw[r] = g < nb * NW ? pw[(size_t)locate(b0 + g / NW) * NW + g % NW] : T(0);
+    for (int r = 0; r < RWD; ++r) {
+      const int e = lane + 64 * r;
+      if (e < nb * NW * Q) {
+        const int k = e / (NW * Q);
+        const unsigned off = unsigned(locate(b0 + k)) * unsigned(NW * Q) + unsigned(e - k * NW * Q);
+        __builtin_amdgcn_global_load_lds(pw_d + off, reinterpret_cast<char *>(sw) + 256 * r, 4, 0, 0);
       }
+    }
 #pragma unroll
-      for (int r = 0; r < RF; ++r) {
-        const int g = lane + 64 * r;
-        vf[r] = g < nb * CC ? fs[(size_t)locate(b0 + g / CC) * C + cc0 + g % CC] : T(0);
+    for (int r = 0; r < RFD; ++r) {
+      const int e = lane + 64 * r;
+      if (e < nb * CC * Q) {
+        const int k = e / (CC * Q);
+        const unsigned off =
+            (unsigned(locate(b0 + k)) * unsigned(C) + unsigned(cc0)) * unsigned(Q) +
+            unsigned(e - k * CC * Q);
+        __builtin_amdgcn_global_load_lds(fs_d + off, reinterpret_cast<char *>(sf) + 256 * r, 4, 0, 0);
       }
-      if (lane < nb) s_pb[lane] = vpb;
+    }
+  };
+  // staged particles k < nb of buffer B whose stencil covers this lane's column
+  auto covers = [&](auto bc, int nb) {
+    constexpr int B = decltype(bc)::value;
+    const int *spb = B ? s_pb1 : s_pb0;
+    Mask m = 0;
+    for (int k = 0; k < nb; ++k) {
+      const int pb = spb[k];
+      int dx = x - (pb & 1023);
+      dx += dx < 0 ? K : 0;
+      int dy = y - ((pb >> 10) & 1023);
+      dy += dy < 0 ? K : 0;
+      m |= (dx < P && dy < P) ? ONE << k : Mask(0);
+    }
+    return m;
+  };
+
+  for (int zb = z0; zb < zend; zb += MAXZ) {
+    const int zs = zb - (P - 1);
+    const int nsteps = min(zb + MAXZ, zend) - zs;
+    for (int q = lane; q < NTL * nsteps; q += 64) {
+      const int t = q / nsteps, st = q - t * nsteps;
+      int tyy = ty - t / NXR, txx = tx - t % NXR;
+      tyy += tyy < 0 ? nty : 0;
+      txx += txx < 0 ? ntx : 0;
+      int bz = zs + st;
+      bz += bz < 0 ? K : 0;
+      const int key = (tyy * ntx + txx) * K + bz;
+      const int lo = boff[key];
+      s_lo[q] = lo;
+      s_n[q] = boff[key + 1] - lo;
+    }
+
+    // bucket offsets are wave-uniform: keep them in scalar registers
+    auto lo_of = [&](int l) { return __builtin_amdgcn_readfirstlane(s_lo[l]); };
+    auto n_of = [&](int l) { return __builtin_amdgcn_readfirstlane(s_n[l]); };
+    // group starting at step st: its ranges, and, per tile, the offset of
+    // the tile's particles in the staging order minus its first sorted
+    // index; true if it is one batch
+    auto plan = [&](int st, int (&glo)[NTL], int (&gcnt)[NTL], int (&gdel)[NTL], int &tot) {
+      tot = 0;
+      if (st >= nsteps) return false;
+      const int gn = min(G, nsteps - st), bz0 = zs + st;
 #pragma unroll
-      for (int r = 0; r < RW; ++r)
-        if (lane + 64 * r < MAXB * NW) s_w[lane + 64 * r] = vw[r];
-#pragma unroll
-      for (int r = 0; r < RF; ++r)
-        if (lane + 64 * r < MAXB * CC) s_f[lane + 64 * r] = vf[r];
-      __syncthreads();
-      // the batch's particles whose stencil covers this lane's column; the
-      // wave then runs as many rounds as its busiest lane has particles, each
-      // lane taking its own next particle (ascending, as a plain loop would)
-      unsigned mine = 0;
-      for (int k = 0; k < nb; ++k) {
-        const int pb = s_pb[k];
-        int dx = x - (pb & 1023);
-        dx += dx < 0 ? K : 0;
-        int dy = y - ((pb >> 10) & 1023);
-        dy += dy < 0 ? K : 0;
-        mine |= (dx < P && dy < P) ? 1u << k : 0u;
+      for (int t = 0; t < NTL; ++t) {
+        const int l = t * nsteps + st;
+        glo[t] = lo_of(l);
+        gcnt[t] = lo_of(l + gn - 1) + n_of(l + gn - 1) - glo[t];
+        gdel[t] = tot - glo[t];
+        tot += gcnt[t];
       }
-      while (mine) {
-        const int k = __ffs(mine) - 1;
-        mine &= mine - 1;
-        const int pb = s_pb[k];
-        int dx = x - (pb & 1023);
-        dx += dx < 0 ? K : 0;
-        int dy = y - ((pb >> 10) & 1023);
-        dy += dy < 0 ? K : 0;
-        const T *w = s_w + k * NW;
-        const T wxy = w[dx] * w[P + dy];
-        T wf[CC];
+      return !(bz0 < 0 && bz0 + gn > 0) && tot <= MAXB;
+    };
+    // the group being copied (n*) and the current one (c*)
+    int nlo[NTL], ncnt[NTL], ndel[NTL], ntot;
+    bool nok = plan(0, nlo, ncnt, ndel, ntot);
+    if (nok) stage(std::integral_constant<int, 0>{}, nlo, ncnt, 0, ntot);
+    bool cok = false;
+    int cdel[NTL];
+    Mask cmask = 0;
+
+    T acc[P][CC];
 #pragma unroll
-        for (int c = 0; c < CC; ++c) wf[c] = wxy * s_f[k * CC + c];
+    for (int j = 0; j < P; ++j)
 #pragma unroll
-        for (int j = 0; j < P; ++j) {
-          const T wz = w[2 * P + j];
+      for (int c = 0; c < CC; ++c) acc[j][c] = T(0);
+
+    // Step st adds the particles whose stencil starts at plane zs + st to the
+    // planes zs + st + j, j < P, held in acc[(st + j) % P]; after the step,
+    // plane zs + st (slot st % P) is complete: it is written and its slot
+    // cleared for plane zs + st + P.  The steps are unrolled by P so that
+    // every slot index and staging buffer is a constant.
+    auto step = [&](int st, auto uc) {
+      constexpr int u = decltype(uc)::value;  // st % P
+      constexpr int B = (u / G) & 1;          // this group's buffer
+      const std::integral_constant<int, B> bc{};
+      const int *spb = B ? s_pb1 : s_pb0;
+      const T *sw = B ? s_w1 : s_w0;
+      const T *sf = B ? s_f1 : s_f0;
+      // the wave runs as many rounds as its busiest lane has particles, each
+      // lane taking its own next covering particle (ascending)
+      auto run = [&](Mask m) {
+        while (m) {
+          const int k = sizeof(Mask) == 8 ? __builtin_ctzll(m) : __builtin_ctz(unsigned(m));
+          m &= m - 1;
+          const int pb = spb[k];
+          int dx = x - (pb & 1023);
+          dx += dx < 0 ? K : 0;
+          int dy = y - ((pb >> 10) & 1023);
+          dy += dy < 0 ? K : 0;
+          const T *w = sw + k * NW;
+          const T wxy = w[dx] * w[P + dy];
+          T wf[CC];
 #pragma unroll
-          for (int c = 0; c < CC; ++c) acc[(u + j) % P][c] += wz * wf[c];
+          for (int c = 0; c < CC; ++c) wf[c] = wxy * sf[k * CC + c];
+#pragma unroll
+          for (int j = 0; j < P; ++j) {
+            const T wz = w[2 * P + j];
+#pragma unroll
+            for (int c = 0; c < CC; ++c) acc[(u + j) % P][c] += wz * wf[c];
+          }
+        }
+      };
+      if constexpr (u % G == 0) {
+        // the copied group becomes current (its copy, issued a group ago,
+        // is waited for); copy the next one into the other buffer, which
+        // the group before the current one has finished reading
+        wait_vm_all();
+        cok = nok;
+        const int ctot = ntot;
+#pragma unroll
+        for (int t = 0; t < NTL; ++t) cdel[t] = ndel[t];
+        nok = plan(st + G, nlo, ncnt, ndel, ntot);
+        if (nok) stage(std::integral_constant<int, 1 - B>{}, nlo, ncnt, 0, ntot);
+        if (cok) cmask = covers(bc, ctot);
+      }
+      int lo[NTL], cnt[NTL], n = 0;
+      Mask sb = 0;  // this step's particles in the staged group
+#pragma unroll
+      for (int t = 0; t < NTL; ++t) {
+        const int l = t * nsteps + st;
+        cnt[t] = n_of(l);
+        lo[t] = lo_of(l);
+        n += cnt[t];
+        if (cok && cnt[t] > 0) {  // offset + count <= MAXB
+          const Mask ones = cnt[t] >= MAXB ? ~Mask(0) : (ONE << cnt[t]) - 1;
+          sb |= ones << (cdel[t] + lo[t]);
         }
       }
-      __syncthreads();
-    }
-    if (st >= P - 1) {
-      T *g = grid + ((size_t)(zs + st) * K + y) * RS + x + (size_t)cc0 * comp_stride;
+      const int nbatch = cok ? 1 : (n + MAXB - 1) / MAXB;
+      for (int b = 0; b < nbatch; ++b) {
+        Mask m;
+        if (cok) {
+          m = cmask & sb;
+        } else {
+          const int nb = min(MAXB, n - b * MAXB);
+          stage(bc, lo, cnt, b * MAXB, nb);
+          wait_vm_all();
+          m = covers(bc, nb);
+        }
+        run(m);
+      }
+      if (st >= P - 1) {
+        T *g = grid + ((size_t)(zs + st) * K + y) * RS + x + (size_t)cc0 * comp_stride;
 #pragma unroll
-      for (int c = 0; c < CC; ++c) g[(size_t)c * comp_stride] = acc[u][c];
-    }
+        for (int c = 0; c < CC; ++c) g[(size_t)c * comp_stride] = acc[u][c];
+      }
 #pragma unroll
-    for (int c = 0; c < CC; ++c) acc[u][c] = T(0);
-  };
-  for (int st0 = 0; st0 < nsteps; st0 += P) unroll_steps<0, P>(st0, nsteps, step);
+      for (int c = 0; c < CC; ++c) acc[u][c] = T(0);
+    };
+    for (int st0 = 0; st0 < nsteps; st0 += P) unroll_steps<0, P>(st0, nsteps, step);
+  }
+  wait_vm_all();
 }
 
 // ---------------------------------------------------------------------------
