@@ -312,9 +312,9 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
 // planes holds what the particles of one phase need, and the S new planes and the
 // particles of the next phase are loaded into registers while the current
 // phase is processed (two barriers per phase).  Each particle is handled by
-// one wave, lanes = the P x P stencil columns; the per-lane products are
-// reduced over the wave for all CC components at once and written without
-// atomics.  Particles of a phase are the contiguous slot ranges of the
+// one 16-lane row of a wave (four particles per wave), a lane summing one
+// stencil column over P/2 stencil rows; the row's partial sums are reduced
+// for all CC components at once and written without atomics.  Particles of a phase are the contiguous slot ranges of the
 // 8 x 8 sort tiles inside the block tile.
 
 // Wave-wide sums of CC values per lane (DPP lane moves and two cross-row
@@ -360,6 +360,27 @@ __device__ __forceinline__ T reduce_components(const T (&v)[CC], int lane) {
     c += dpp_mov<0x4E>(c);  // lanes with equal parity of the quad
     return sum_same_residue4(c);
   }
+}
+
+// The same over the 16 lanes of a DPP row only: every lane gets the total
+// of component (lane % 4) over its own row.
+template <typename T, int CC>
+__device__ __forceinline__ T reduce_components_row(const T (&v)[CC], int lane) {
+  static_assert(CC == 4 || CC == 2, "4 FP32 or 2 FP64 components");
+  const bool o = lane & 1;
+  T c;
+  if constexpr (CC == 4) {
+    const T a = (o ? v[1] : v[0]) + dpp_mov<0xB1>(o ? v[0] : v[1]);
+    const T b = (o ? v[3] : v[2]) + dpp_mov<0xB1>(o ? v[2] : v[3]);
+    const bool h = lane & 2;
+    c = (h ? b : a) + dpp_mov<0x4E>(h ? a : b);
+  } else {
+    c = (o ? v[1] : v[0]) + dpp_mov<0xB1>(o ? v[0] : v[1]);
+    c += dpp_mov<0x4E>(c);
+  }
+  c += dpp_mov<0x124>(c);
+  c += dpp_mov<0x128>(c);
+  return c;
 }
 
 template <typename T> struct alignas(16) Vec16 {
@@ -538,44 +559,55 @@ __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
     }
   };
 
-  const int dx = lane & 7, dy = lane >> 3;
-  const bool act = dx < P && dy < P;
-  const int dxc = min(dx, P - 1), dyc = min(dy, P - 1);
+  // Four particles per wave, one per 16-lane DPP row: lane = (row g, h,
+  // dx); the lane sums stencil column dx of stencil rows 2i + h, i < P/2
+  // (its two rows of a 16-lane LDS phase are adjacent: disjoint bank
+  // halves), and the row's 16 lanes are reduced with DPP moves only.
+  constexpr int NR = P / 2;
+  const int g = lane >> 4, h = (lane >> 3) & 1, dx = lane & 7;
+  const bool act = dx < P;
+  const int dxc = min(dx, P - 1);
   auto process = [&](int nb) {
-    for (int k = wave; k < nb; k += NWV) {
-      const int pb = s_pb[k];
+    for (int k0 = 4 * wave; k0 < nb; k0 += 4 * NWV) {
+      const int k = k0 + g;
+      const bool kon = k < nb;
+      const int kk = kon ? k : k0;  // a staged particle (loads stay in range)
+      const int pb = s_pb[kk];
       const int rx = (pb & 1023) - x0, ry = ((pb >> 10) & 1023) - y0;
       const int bz = pb >> 20;  // in [z0, zend)
-      const T *w = s_w + k * NW;
-      const T wxy = act ? w[dxc] * w[P + dyc] : T(0);
+      const T *w = s_w + kk * NW;
+      const T wx = act ? w[dxc] : T(0);
+      // window points of plane j of this lane's column (first row h)
+      const int sl0 = bz & (NSLOT - 1);
+      const V *pt = win + (ry + h) * EP + rx + dxc;
+      const V *pj[P];
+#pragma unroll
+      for (int j = 0; j < P; ++j) pj[j] = pt + ((sl0 + j) & (NSLOT - 1)) * PLANE;
       T acc[CC];
 #pragma unroll
       for (int c = 0; c < CC; ++c) acc[c] = T(0);
-      const int sl0 = bz & (NSLOT - 1);
-      const V *pt = win + (ry + dyc) * EP + rx + dxc;
-      if (sl0 + P <= NSLOT) {  // the particle's planes sit in consecutive slots
-        const V *pz = pt + sl0 * PLANE;
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        T ar[CC];
+#pragma unroll
+        for (int c = 0; c < CC; ++c) ar[c] = T(0);
 #pragma unroll
         for (int j = 0; j < P; ++j) {
           const T wz = w[2 * P + j];
-          const V v = pz[j * PLANE];
+          const V v = pj[j][2 * i * EP];
 #pragma unroll
-          for (int c = 0; c < CC; ++c) acc[c] += wz * v.v[c];
+          for (int c = 0; c < CC; ++c) ar[c] += wz * v.v[c];
         }
-      } else {
+        const T wy = w[P + 2 * i + h];
 #pragma unroll
-        for (int j = 0; j < P; ++j) {
-          const T wz = w[2 * P + j];
-          const V v = pt[((sl0 + j) & (NSLOT - 1)) * PLANE];
-#pragma unroll
-          for (int c = 0; c < CC; ++c) acc[c] += wz * v.v[c];
-        }
+        for (int c = 0; c < CC; ++c) acc[c] += wy * ar[c];
       }
 #pragma unroll
-      for (int c = 0; c < CC; ++c) acc[c] *= wxy;
-      const T tot = reduce_components<T, CC>(acc, lane);
-      if (lane < ncc) {
-        const int comp = cc0 + lane, r = comp / 3, d = comp - 3 * r;
+      for (int c = 0; c < CC; ++c) acc[c] *= wx;
+      const T tot = reduce_components_row<T, CC>(acc, lane);
+      const int cl = lane & 15;
+      if (kon && cl < ncc) {
+        const int comp = cc0 + cl, r = comp / 3, d = comp - 3 * r;
         T *u = out + r * ldout + 3L * s_perm[k] + d;
         *u = beta == T(0) ? alpha * tot : beta * *u + alpha * tot;
       }
