@@ -842,7 +842,10 @@ __device__ __forceinline__ void fft_y_block(T *__restrict__ grid, long comp_stri
 // Blocks follow ZLayout: per right-hand side, (x chunk, y) blocks of XW
 // adjacent columns, then (when K/2 is a multiple of XW) Nyquist blocks whose
 // XW lanes take column x = K/2 of XW consecutive rows y.
-// tab[(z*K + y)*Khp + x] = psi(|k|)|b(k)|^2/V.
+// m2(k) = psi(|k|)|b(k)|^2/V is formed from FP64-built factors:
+// tab[n] = psi(dk sqrt(n))/V for n = nx^2 + ny^2 + nz^2 (integer wave
+// numbers) and tab[NPSI + m] = |b(m)|^2, so the z pass reads a table of
+// O(K^2) entries that stays in cache instead of one of K^2 (K/2 + 1).
 // Wave numbers follow old-version/spme.c:95-107 (index K/2 -> +pi K/L).
 // The spectrum of a (column, k1) stays in registers between the forward
 // N2-point FFT, the 3x3 influence and the backward N2-point FFT.
@@ -888,11 +891,18 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
   if (threadIdx.x < XW * N1) {  // step B, influence, inverse step B
     const int xo = threadIdx.x % XW, k1 = threadIdx.x / XW;
     const int x = xa + xo * dx, y = ya + xo * dy;
-    // k = 0 and padding columns carry m2 = 0
+    // k = 0 (tab[0] = 0) and padding columns (x > K/2) carry m2 = 0
+    constexpr int NPSI = influence_npsi(K);
+    const T *bsq = tab + NPSI;
+    const int nx = min(x, K / 2), ny = y <= K / 2 ? y : K - y;
+    const int nxy = nx * nx + ny * ny;
+    const T bxy = x < Kh ? bsq[nx] * bsq[y] : T(0);
     T m2[N2];
 #pragma unroll
-    for (int k2 = 0; k2 < N2; ++k2)
-      m2[k2] = x < Kh ? tab[((size_t)(k1 + N1 * k2) * K + y) * Khp + x] : T(0);
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const int z = k1 + N1 * k2, nz = z <= K / 2 ? z : K - z;
+      m2[k2] = tab[nxy + nz * nz] * (bxy * bsq[z]);
+    }
     cpx<T> b[3][N2];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {

@@ -141,7 +141,7 @@ struct _spme_t {
   bool have_pos;
   // mesh and tables
   void *grid;  // 3*max_rhs components
-  void *tab;   // influence table [K][K][Khp] in mesh precision
+  void *tab;   // plain path: influence table [K][K][Khp]; otherwise its factors
   void *tw;    // twiddles exp(-2 pi i t/K), K complex
   // P data
   int nkeys;
@@ -191,11 +191,17 @@ int upload_tables(_spme_t *e) {
   double *d_bsq = nullptr;
   HIP_TRY(hipMalloc(&d_bsq, sizeof(double) * K));
   HIP_TRY(hipMemcpy(d_bsq, bsq.data(), sizeof(double) * K, hipMemcpyHostToDevice));
-  HIP_TRY(hipMalloc(&e->tab, sizeof(T) * (size_t)K * K * e->Khp));
-  // the plain path keeps the reference's lm2 = psi |b|^2 and scales by
-  // 1/K^3 and alpha K^3/V in passes of their own; the others fold 1/V here
-  launch_influence_table<T>(d_bsq, K, e->Khp, 2.0 * M_PI / L, xi, e->dense ? 1.0 : 1.0 / V,
-                            (T *)e->tab, e->stream);
+  if (e->dense) {
+    // the plain path keeps the reference's lm2 = psi |b|^2 table and scales
+    // by 1/K^3 and alpha K^3/V in passes of their own
+    HIP_TRY(hipMalloc(&e->tab, sizeof(T) * (size_t)K * K * e->Khp));
+    launch_influence_table<T>(d_bsq, K, e->Khp, 2.0 * M_PI / L, xi, 1.0, (T *)e->tab,
+                              e->stream);
+  } else {
+    // the z pass forms psi |b|^2 / V from its factors
+    HIP_TRY(hipMalloc(&e->tab, sizeof(T) * (size_t)(influence_npsi(K) + K)));
+    launch_influence_factors<T>(d_bsq, K, 2.0 * M_PI / L, xi, 1.0 / V, (T *)e->tab, e->stream);
+  }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(e->stream));
   HIP_TRY(hipFree(d_bsq));

@@ -567,6 +567,30 @@ __global__ __launch_bounds__(256) void k_influence_table(const double *__restric
   tab[t] = T(v);
 }
 
+// Factors of the same table for the z pass, FP64-built, stored in mesh
+// precision: fac[n] = psi(dk sqrt(n)) inv_v for n = nx^2 + ny^2 + nz^2 <=
+// 3 (K/2)^2 (the wave numbers are integer multiples of dk), then |b(m)|^2
+// for m < K; the z pass forms psi |b_x|^2 |b_y|^2 |b_z|^2 / V from them.
+template <typename T>
+__global__ __launch_bounds__(256) void k_influence_factors(const double *__restrict__ bsq,
+                                                           int K, double dk, double xi,
+                                                           double inv_v, T *__restrict__ fac) {
+  const int npsi = influence_npsi(K);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < npsi) {
+    const double k2 = double(t) * dk * dk;
+    double v = 0.0;
+    if (t > 0) {
+      const double q = k2 / (xi * xi);
+      v = 6.0 * M_PI * exp(-q / 4.0) * (1.0 - k2 / 3.0) / k2 * (1.0 + q * (0.25 + q / 8.0)) *
+          inv_v;
+    }
+    fac[t] = T(v);
+  } else if (t < npsi + K) {
+    fac[t] = T(bsq[t - npsi]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers.
 
@@ -577,6 +601,17 @@ void launch_influence_table(const double *bsq, int K, int Khp, double dk, double
   hipLaunchKernelGGL(k_influence_table<T>, dim3(int((n + 255) / 256)), dim3(256), 0, st, bsq, K,
                      Khp, dk, xi, inv_v, tab);
 }
+template <typename T>
+void launch_influence_factors(const double *bsq, int K, double dk, double xi, double inv_v,
+                              T *fac, hipStream_t st) {
+  const int n = influence_npsi(K) + K;
+  hipLaunchKernelGGL(k_influence_factors<T>, dim3((n + 255) / 256), dim3(256), 0, st, bsq, K, dk,
+                     xi, inv_v, fac);
+}
+template void launch_influence_factors<float>(const double *, int, double, double, double,
+                                              float *, hipStream_t);
+template void launch_influence_factors<double>(const double *, int, double, double, double,
+                                               double *, hipStream_t);
 template void launch_influence_table<float>(const double *, int, int, double, double, double,
                                             float *, hipStream_t);
 template void launch_influence_table<double>(const double *, int, int, double, double, double,

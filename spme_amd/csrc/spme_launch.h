@@ -90,11 +90,14 @@ template <typename T, int K> struct ZShape {
   static constexpr int NT = wide ? 512 : 256;
 };
 
+// Entries of the psi factor table: n = nx^2 + ny^2 + nz^2 <= 3 (K/2)^2.
+__host__ __device__ constexpr int influence_npsi(int K) { return 3 * (K / 2) * (K / 2) + 1; }
+
 template <typename T> struct FftArgs {
   T *grid;
   long comp_stride;
   int RS, K, ncomp;
-  const T *tab;
+  const T *tab;  // influence factors (psi over n^2, then |b|^2), see the z pass
   T dk;
   const cpx<T> *tw;
   int ncu;  // compute units (grid of the fused plane passes)
@@ -146,5 +149,8 @@ bool fft_size_supported(int K);
 template <typename T>
 void launch_influence_table(const double *bsq, int K, int Khp, double dk, double xi,
                             double inv_v, T *tab, hipStream_t st);
+template <typename T>
+void launch_influence_factors(const double *bsq, int K, double dk, double xi, double inv_v,
+                              T *fac, hipStream_t st);
 
 }  // namespace spme
