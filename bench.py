@@ -132,7 +132,8 @@ def algorithmic_bytes(N, K, p, R, s, real_entries):
         "spread": vec + N * bP + K3 * C * s,
         # fused x + y passes: real mesh in, half spectrum out (and back)
         "fft_x_fwd": K3 * C * s + Kh * C * 2 * s,
-        "fft_z_infl": 2 * Kh * C * 2 * s + Kh * s,
+        # the influence factors (O(K^2) values) stay in cache
+        "fft_z_infl": 2 * Kh * C * 2 * s,
         "fft_x_inv": Kh * C * 2 * s + K3 * C * s,
         "interp": K3 * C * s + N * bP + vec,
         "permute_real": 2 * vec + 4 * N,

@@ -6,10 +6,12 @@
 //   gram:   C[a][b] = sum_i X[a][i] Y[b][i]        (FP64 accumulation)
 //   update: Y[b][i] = beta Y[b][i] + sum_a X[a][i] A[a][b]
 //
-// The gram is deterministic: per-block partial sums in a workspace, then a
-// fixed-order reduction.
+// The gram is deterministic: a fixed number of blocks (a function of n
+// only) each sum a fixed set of chunks in a fixed order into a workspace of
+// partial sums, then one block per entry reduces them in a fixed tree.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 
@@ -27,39 +29,50 @@ int fail(int code, const char *fmt, ...) {
   return set_last_error(code, buf);
 }
 
-constexpr int BS = 16;     // largest block width
-constexpr int GCH = 256;   // vector entries per gram block
+constexpr int BS = 16;      // largest block width
+constexpr int GCH = 256;    // vector entries per staged chunk
+constexpr int GMAXB = 512;  // most partial-sum blocks
 
+int gram_blocks(long n) { return int(std::min<long>((n + GCH - 1) / GCH, GMAXB)); }
+
+// Block g sums chunks g, g + nb, g + 2 nb, ... of the vectors; thread
+// (a, b) keeps its entry's partial sum in a register across chunks.
 template <typename T>
 __global__ __launch_bounds__(256) void k_block_gram_part(long n, int sx, int sy,
                                                          const T *__restrict__ X, long ldx,
                                                          const T *__restrict__ Y, long ldy,
                                                          double *__restrict__ part) {
-  __shared__ double xs[BS][GCH + 1], ys[BS][GCH + 1];
-  const long i0 = (long)blockIdx.x * GCH;
-  const int m = int(n - i0 < GCH ? n - i0 : GCH);
-  for (int t = threadIdx.x; t < BS * GCH; t += blockDim.x) {
-    const int r = t / GCH, k = t - r * GCH;
-    xs[r][k] = (r < sx && k < m) ? double(X[r * ldx + i0 + k]) : 0.0;
-    ys[r][k] = (r < sy && k < m) ? double(Y[r * ldy + i0 + k]) : 0.0;
-  }
-  __syncthreads();
+  __shared__ T xs[BS][GCH + 1], ys[BS][GCH + 1];
   const int a = threadIdx.x / BS, b = threadIdx.x % BS;
-  if (a < sx && b < sy) {
-    double s = 0.0;
-    for (int k = 0; k < m; ++k) s = fma(xs[a][k], ys[b][k], s);
-    part[(size_t)blockIdx.x * (sx * sy) + a * sy + b] = s;
+  const long nch = (n + GCH - 1) / GCH;
+  double s = 0.0;
+  for (long c = blockIdx.x; c < nch; c += gridDim.x) {
+    const long i0 = c * GCH;
+    const int m = int(n - i0 < GCH ? n - i0 : GCH);
+    __syncthreads();  // the previous chunk has been read
+    for (int t = threadIdx.x; t < BS * GCH; t += blockDim.x) {
+      const int r = t / GCH, k = t - r * GCH;
+      xs[r][k] = (r < sx && k < m) ? X[r * ldx + i0 + k] : T(0);
+      ys[r][k] = (r < sy && k < m) ? Y[r * ldy + i0 + k] : T(0);
+    }
+    __syncthreads();
+    if (a < sx && b < sy)
+      for (int k = 0; k < m; ++k) s = fma(double(xs[a][k]), double(ys[b][k]), s);
   }
+  if (a < sx && b < sy) part[(size_t)blockIdx.x * (sx * sy) + a * sy + b] = s;
 }
 
-__global__ __launch_bounds__(256) void k_block_gram_reduce(int nb, int ss,
-                                                           const double *__restrict__ part,
-                                                           double *__restrict__ C) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ss) return;
+// Entry e = blockIdx.x: lane l sums partials l, l + 64, ... in order, then a
+// fixed butterfly over the wave.
+__global__ __launch_bounds__(64) void k_block_gram_reduce(int nb, int ss,
+                                                          const double *__restrict__ part,
+                                                          double *__restrict__ C) {
+  const int e = blockIdx.x, l = threadIdx.x;
   double s = 0.0;
-  for (int k = 0; k < nb; ++k) s += part[(size_t)k * ss + e];
-  C[e] = s;
+  for (int k = l; k < nb; k += 64) s += part[(size_t)k * ss + e];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  if (l == 0) C[e] = s;
 }
 
 template <typename T>
@@ -103,7 +116,7 @@ using namespace spme;
 extern "C" {
 
 long spme_block_gram_worksize(long n, int sx, int sy) {
-  return ((n + GCH - 1) / GCH) * (long)sx * sy;
+  return (long)gram_blocks(n) * sx * sy;
 }
 
 int spme_block_gram(int precision, long n, int sx, int sy, const void *X, long ldx,
@@ -115,14 +128,14 @@ int spme_block_gram(int precision, long n, int sx, int sy, const void *X, long l
   if (work_len < need) return fail(-1, "workspace too small (%ld < %ld doubles)", work_len, need);
   (void)hipGetLastError();
   hipStream_t st = (hipStream_t)stream;
-  const int nb = int((n + GCH - 1) / GCH);
+  const int nb = gram_blocks(n);
   if (precision == SPME_FP32)
     hipLaunchKernelGGL(k_block_gram_part<float>, dim3(nb), dim3(256), 0, st, n, sx, sy,
                        (const float *)X, ldx, (const float *)Y, ldy, work);
   else
     hipLaunchKernelGGL(k_block_gram_part<double>, dim3(nb), dim3(256), 0, st, n, sx, sy,
                        (const double *)X, ldx, (const double *)Y, ldy, work);
-  hipLaunchKernelGGL(k_block_gram_reduce, dim3(1), dim3(256), 0, st, nb, sx * sy, work, C);
+  hipLaunchKernelGGL(k_block_gram_reduce, dim3(sx * sy), dim3(64), 0, st, nb, sx * sy, work, C);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : fail(-2, "block gram launch: %s", hipGetErrorString(e));
 }

@@ -51,17 +51,27 @@ class HipBlockOps:
     def _stream(self):
         return ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
 
-    def gram(self, X, Y) -> np.ndarray:
+    def gram_dev(self, X, Y, out=None):
+        """X Y^T as an FP64 device tensor [sx][sy], without waiting for it."""
         sx, sy = X.shape[0], Y.shape[0]
+        C = out if out is not None else self.torch.empty(sx * sy, dtype=self.torch.float64,
+                                                         device=self.device)
         check(load().spme_block_gram(self.precision, self.n, sx, sy, _ptr(X), X.stride(0),
-                                     _ptr(Y), Y.stride(0), _ptr(self._C), _ptr(self._work),
+                                     _ptr(Y), Y.stride(0), _ptr(C), _ptr(self._work),
                                      self._work.numel(), self._stream()), "spme_block_gram")
-        return self._C[:sx * sy].cpu().numpy().reshape(sx, sy).copy()
+        return C[:sx * sy].view(sx, sy)
 
-    def update(self, X, A: np.ndarray, beta: float, Y) -> None:
+    def gram(self, X, Y) -> np.ndarray:
+        return self.gram_dev(X, Y, self._C).cpu().numpy().copy()
+
+    def update(self, X, A, beta: float, Y) -> None:
+        """Y = beta Y + A^T X in the rows' sense; A: numpy or FP64 device tensor."""
         sx, sy = X.shape[0], Y.shape[0]
-        A = np.ascontiguousarray(A, dtype=np.float64).reshape(sx, sy)
-        dA = self.torch.tensor(A, device=self.device)
+        if self.torch.is_tensor(A):
+            dA = A.reshape(sx, sy).to(self.torch.float64).contiguous()
+        else:
+            A = np.ascontiguousarray(A, dtype=np.float64).reshape(sx, sy)
+            dA = self.torch.tensor(A, device=self.device)
         check(load().spme_block_update(self.precision, self.n, sx, sy, _ptr(X), X.stride(0),
                                        _ptr(dA), float(beta), _ptr(Y), Y.stride(0),
                                        self._stream()), "spme_block_update")
@@ -88,8 +98,22 @@ class LanczosInfo:
     invariant: bool      # the space became invariant (the result is exact)
 
 
+def _eigh(T: np.ndarray):
+    """Eigendecomposition of the symmetric T; on the GPU (FP64) once T is
+    large enough for the host's O(n^3) to matter."""
+    if T.shape[0] >= 512:
+        try:
+            import torch
+            if torch.cuda.is_available():
+                w, Q = torch.linalg.eigh(torch.tensor(T, device="cuda"))
+                return w.cpu().numpy(), Q.cpu().numpy()
+        except ImportError:
+            pass
+    return np.linalg.eigh(T)
+
+
 def _sqrt_first_cols(T: np.ndarray, k0: int):
-    w, Q = np.linalg.eigh(0.5 * (T + T.T))
+    w, Q = _eigh(0.5 * (T + T.T))
     if w[0] < -1e-8 * max(abs(w[-1]), 1e-300):
         raise SpmeError(f"operator is not positive definite (Ritz value {w[0]:.3e})")
     r = np.sqrt(np.clip(w, 0.0, None))
@@ -181,8 +205,11 @@ class BlockLanczos:
             As.append(A)
             scale = max(scale, float(np.abs(np.linalg.eigvalsh(A)).max()))
             if self.reorth:
+                # W -= Q_i (Q_i^T W), block by block; with device-side Gram
+                # products nothing waits on the host inside this loop
+                dev = getattr(ops, "gram_dev", None)
                 for Qi in Qs:
-                    ops.update(Qi, -ops.gram(Qi, W), 1.0, W)
+                    ops.update(Qi, -(dev(Qi, W) if dev else ops.gram(Qi, W)), 1.0, W)
             m = j + 1
             # the (s m)^3 eigendecomposition is checked every step while it is
             # cheap, then every m/4 steps (the change is then measured over
