@@ -120,8 +120,8 @@ def pmc_run():
 def algorithmic_bytes(N, K, p, R, s, real_entries):
     """Compulsory HBM bytes per launch of each kernel: b_P = 3p*s + 12 per
     particle, Kh = (K/2+1)K^2, C = 3R components; the real-space SpMV reads
-    each stored pair entry once (6 block values + a column index) and each
-    force and output vector once."""
+    each stored pair entry once (2 coefficients + a column index), each
+    position, force and output vector once."""
     C = 3 * R
     K3 = K ** 3
     Kh = (K // 2 + 1) * K * K
@@ -137,7 +137,9 @@ def algorithmic_bytes(N, K, p, R, s, real_entries):
         "fft_x_inv": Kh * C * 2 * s + K3 * C * s,
         "interp": K3 * C * s + N * bP + vec,
         "permute_real": 2 * vec + 4 * N,
-        "real": real_entries * (6 * s + 4) + 3 * vec + 4 * N,
+        # per stored entry: column index and the coefficients F, G/r^2; the
+        # FP64 positions once per particle
+        "real": real_entries * (2 * s + 4) + 3 * vec + 24 * N + 4 * N,
         # the whole forward + inverse transform as one unit
         "fft_total": 2 * (K3 * C * s + Kh * C * 2 * s),
     }

@@ -960,12 +960,15 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
 }
 
 // Real-space SpMV: a row (cell order) is handled by LPR lanes, each with RPL
-// consecutive right-hand sides, so the pair's block and column index are
-// loaded once per RPL right-hand sides and the neighbour's 3 RPL forces come
-// as 16-byte loads.  fs: forces in cell order [N][C].  For right-hand side r
-// and d < 3:
+// consecutive right-hand sides, so the pair's coefficients and column index
+// are loaded once per RPL right-hand sides and the neighbour's 3 RPL forces
+// come as 16-byte loads.  fs: forces in cell order [N][C].  For right-hand
+// side r and d < 3:
 //   out[r*ldout + 3*perm[s] + d] = (accum ? out : beta*out)
-//                                  + alpha*(self f + sum_j B_sj f_j).
+//                                  + alpha*(self f + sum_j B_sj f_j),
+// B_sj = F I + (G/r^2) e e^T with e the minimum-image separation, formed
+// from the stored FP64-built F and G/r^2 and the FP64 positions (8 stored
+// bytes per entry instead of 6 block values).
 // Rows are packed in slices of 64 (entry-major within a slice); the rows of
 // a wave lie in one slice, so every lane runs the same entry count.
 template <int C> struct RealShape {
@@ -997,7 +1000,8 @@ __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
                                           const int *__restrict__ width,
                                           const int *__restrict__ soff,
                                           const int *__restrict__ col,
-                                          const T *__restrict__ blk, long E, T self,
+                                          const T *__restrict__ blk, long E,
+                                          const double *__restrict__ spos, double L, T self,
                                           const T *__restrict__ fs, T alpha, T beta, int accum,
                                           T *__restrict__ out, long ldout, int s, int l) {
   constexpr int RPL = RealShape<C>::RPL, NV = 3 * RPL;
@@ -1007,14 +1011,23 @@ __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
   load_n<T, NV>(fs + (size_t)s * C + 3 * r0, f);
 #pragma unroll
   for (int i = 0; i < NV; ++i) a[i] = self * f[i];
+  const double px = spos[3L * s], py = spos[3L * s + 1], pz = spos[3L * s + 2];
+  const double iL = 1.0 / L;
   const long base = soff[q] + ls;
   const int w = width[q];
 #pragma unroll 4
   for (int k = 0; k < w; ++k) {
     const long e = base + 64L * k;
     const int j = col[e];
-    const T b0 = blk[e], b1 = blk[E + e], b2 = blk[2 * E + e];
-    const T b3 = blk[3 * E + e], b4 = blk[4 * E + e], b5 = blk[5 * E + e];
+    const T F = blk[e], g = blk[E + e];
+    double ex = spos[3L * j] - px, ey = spos[3L * j + 1] - py, ez = spos[3L * j + 2] - pz;
+    ex -= L * rint(ex * iL);
+    ey -= L * rint(ey * iL);
+    ez -= L * rint(ez * iL);
+    const T x = T(ex), y = T(ey), z = T(ez);
+    const T gx = g * x, gy = g * y, gz = g * z;
+    const T b0 = F + gx * x, b1 = gx * y, b2 = gx * z;
+    const T b3 = F + gy * y, b4 = gy * z, b5 = F + gz * z;
     load_n<T, NV>(fs + (size_t)j * C + 3 * r0, f);
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {

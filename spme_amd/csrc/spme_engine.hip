@@ -383,7 +383,7 @@ int build_P(_spme_t *e, hipStream_t st) {
       e->r_cap = 0;
       const long cap = total + total / 8 + 64;
       HIP_TRY(hipMalloc(&e->r_col, sizeof(int) * cap));
-      HIP_TRY(hipMalloc(&e->r_blk, sizeof(T) * 6 * cap));
+      HIP_TRY(hipMalloc(&e->r_blk, sizeof(T) * 2 * cap));
       e->r_cap = cap;
     }
     e->r_E = e->r_cap;
@@ -559,8 +559,8 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
                                   40.0 * xi * xi * xi / (3.0 * std::sqrt(M_PI))
                             : 0.0;
     RealArgs<T> ra{e->r_perm, e->r_width, e->r_soff, e->r_col, (const T *)e->r_blk, e->r_E,
-                   self, (const T *)e->r_fs, N, C, alpha, beta, (parts & SPME_RECIP) ? 1 : 0,
-                   out, ldout};
+                   e->r_pos, e->prm.box_size, self, (const T *)e->r_fs, N, C, alpha, beta,
+                   (parts & SPME_RECIP) ? 1 : 0, out, ldout};
     if (launch_real<T>(ra, st)) return fail(-1, "real space: unsupported batch width");
     mark(SPME_ST_REAL);
   }
@@ -844,8 +844,8 @@ int group_build_args(_spme_group_t *g, double alpha, const void *const *in, cons
                                                   40.0 * xi * xi * xi / (3.0 * std::sqrt(M_PI))
                                             : 0.0;
     ra[s] = RealArgs<T>{e->r_perm, e->r_width, e->r_soff, e->r_col, (const T *)e->r_blk,
-                        e->r_E, self, (const T *)e->r_fs, e->N, C, alpha, beta, rec ? 1 : 0,
-                        (T *)out[s], ldout[s]};
+                        e->r_E, e->r_pos, e->prm.box_size, self, (const T *)e->r_fs, e->N, C,
+                        alpha, beta, rec ? 1 : 0, (T *)out[s], ldout[s]};
   }
   HIP_TRY(hipMemcpyAsync(g->d_args, g->h_args, bytes, hipMemcpyHostToDevice, st));
   HIP_TRY(hipEventRecord(g->up_ev, st));

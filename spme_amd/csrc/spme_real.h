@@ -8,9 +8,11 @@ template <typename T> struct RealArgs {
   const int *perm;   // cell order -> original index
   const int *width;  // entries per row, per slice of 64 rows
   const int *soff;   // slice offsets (entries)
-  const int *col;    // neighbour (cell order) or -1
-  const T *blk;      // 6 x E block values
+  const int *col;    // neighbour (cell order)
+  const T *blk;      // 2 x E pair coefficients: F, then G / r^2
   long E;
+  const double *spos;  // positions in cell order [N][3]
+  double L;
   double self;
   const T *fs;  // forces in cell order [N][C]
   int N, C;

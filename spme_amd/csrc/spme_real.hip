@@ -81,9 +81,10 @@ __global__ __launch_bounds__(256) void k_cell_gather(const int *__restrict__ per
 // order, grouped in slices of 64 rows; slice q holds width[q] entries per
 // row, stored entry-major (entry k of row 64q + l at soff[q] + 64k + l) so
 // that a wave reads each entry array coalesced.  Each entry is a neighbour
-// j (cell order) and the 6 unique values of its symmetric 3x3 block
-// M_real + overlap; past the end of a row, the row's own index with a zero
-// block.
+// j (cell order) and the two coefficients F and G/r^2 of its symmetric 3x3
+// block M_real + overlap = F I + (G/r^2) e e^T (the SpMV forms e from the
+// positions); past the end of a row, the row's own index with zero
+// coefficients.
 
 __device__ __forceinline__ int cell_of(double p, double L, int nc) {
   double t = p / L;
@@ -168,15 +169,10 @@ __global__ __launch_bounds__(256) void k_pair_fill(const double *__restrict__ sp
                                                  double r2) {
     double F, G;
     pair_coeffs(r2, xi, rc2, F, G);
-    const double g = G / r2;
     const long e = base + 64L * k;
     col[e] = j;
-    blk[0 * E + e] = T(F + g * ex * ex);
-    blk[1 * E + e] = T(g * ex * ey);
-    blk[2 * E + e] = T(g * ex * ez);
-    blk[3 * E + e] = T(F + g * ey * ey);
-    blk[4 * E + e] = T(g * ey * ez);
-    blk[5 * E + e] = T(F + g * ez * ez);
+    blk[e] = T(F);
+    blk[E + e] = T(G / r2);
     ++k;
   });
   // pad the row with zero blocks on the row's own particle (no branch in the
@@ -184,8 +180,8 @@ __global__ __launch_bounds__(256) void k_pair_fill(const double *__restrict__ sp
   for (; k < width[q]; ++k) {
     const long e = base + 64L * k;
     col[e] = s;
-#pragma unroll
-    for (int m = 0; m < 6; ++m) blk[m * E + e] = T(0);
+    blk[e] = T(0);
+    blk[E + e] = T(0);
   }
 }
 
@@ -196,6 +192,7 @@ __global__ __launch_bounds__(256) void k_real_spmv(const int *__restrict__ perm,
                                                    const int *__restrict__ soff,
                                                    const int *__restrict__ col,
                                                    const T *__restrict__ blk, long E,
+                                                   const double *__restrict__ spos, double L,
                                                    T self, const T *__restrict__ fs, int N,
                                                    T alpha, T beta, int accum,
                                                    T *__restrict__ out, long ldout) {
@@ -203,8 +200,8 @@ __global__ __launch_bounds__(256) void k_real_spmv(const int *__restrict__ perm,
   const long g = (long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int s = int(g / LPR), l = int(g % LPR);
   if (s < N)
-    real_lane<T, C>(perm, width, soff, col, blk, E, self, fs, alpha, beta, accum, out, ldout, s,
-                    l);
+    real_lane<T, C>(perm, width, soff, col, blk, E, spos, L, self, fs, alpha, beta, accum, out,
+                    ldout, s, l);
 }
 
 static int grid1(long n, int b) { return int((n + b - 1) / b); }
@@ -243,7 +240,8 @@ static void real_c(const RealArgs<T> &a, hipStream_t st) {
   hipLaunchKernelGGL((k_real_spmv<T, C>), dim3(grid1((long)a.N * RealShape<C>::LPR, 256)),
                      dim3(256), 0,
                      st, a.perm,
-                     a.width, a.soff, a.col, a.blk, a.E, T(a.self), a.fs, a.N, T(a.alpha),
+                     a.width, a.soff, a.col, a.blk, a.E, a.spos, a.L, T(a.self), a.fs, a.N,
+                     T(a.alpha),
                      T(a.beta), a.accum, a.out, a.ldout);
 }
 
