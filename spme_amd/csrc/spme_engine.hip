@@ -157,6 +157,7 @@ struct _spme_t {
   // real space
   int nc;
   int *r_key, *r_counts, *r_cursor, *r_off, *r_perm;
+  int *iperm, *r_iperm;  // inverses of perm and r_perm (original -> sorted)
   double *r_pos;
   void *r_fs;
   // real-space pair blocks (sliced ELL, 64 rows per slice)
@@ -241,7 +242,7 @@ void free_all(_spme_t *e) {
                   e->cursor, e->boff, e->perm, e->pbase, e->err, e->w_tmp, e->pw, e->fs,
                   e->r_key, e->r_counts, e->r_cursor, e->r_off, e->r_perm, e->r_pos,
                   e->r_fs, e->r_cnt, e->r_width, e->r_size, e->r_soff, e->r_col, e->r_blk,
-                  e->h_in, e->h_out, e->dP, e->dind};
+                  e->h_in, e->h_out, e->dP, e->dind, e->iperm, e->r_iperm};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &r : e->pending) {
@@ -296,6 +297,7 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
   E_TRY(hipMalloc(&e->cursor, sizeof(int) * e->nkeys));
   E_TRY(hipMalloc(&e->boff, sizeof(int) * (e->nkeys + 1)));
   E_TRY(hipMalloc(&e->perm, sizeof(int) * N));
+  E_TRY(hipMalloc(&e->iperm, sizeof(int) * N));
   E_TRY(hipMalloc(&e->pbase, sizeof(int) * N));
   E_TRY(hipMalloc(&e->err, sizeof(int)));
   E_TRY(hipMalloc(&e->w_tmp, sizeof(double) * 3 * e->P * N));
@@ -324,6 +326,7 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
     E_TRY(hipMalloc(&e->r_cursor, sizeof(int) * nc3));
     E_TRY(hipMalloc(&e->r_off, sizeof(int) * (nc3 + 1)));
     E_TRY(hipMalloc(&e->r_perm, sizeof(int) * N));
+    E_TRY(hipMalloc(&e->r_iperm, sizeof(int) * N));
     E_TRY(hipMalloc(&e->r_pos, sizeof(double) * 3 * N));
     E_TRY(hipMalloc(&e->r_fs, e->tsz * (size_t)C * N));
     e->nslices = (N + 63) / 64;
@@ -355,6 +358,7 @@ int build_P(_spme_t *e, hipStream_t st) {
     launch_scan(e->counts, e->boff, e->nkeys, st);
     launch_pscatter(e->key_tmp, e->boff, e->cursor, e->perm, N, st);
     launch_bucket_sort(e->boff, e->perm, e->nkeys, st);
+    launch_invert(e->perm, e->iperm, N, st);
     launch_pgather<T>(e->perm, e->base_tmp, e->w_tmp, e->pbase, (T *)e->pw, N, 3 * P, st);
   }
   if (e->nc >= 3) {
@@ -365,6 +369,7 @@ int build_P(_spme_t *e, hipStream_t st) {
     launch_scan(e->r_counts, e->r_off, nc3, st);
     launch_pscatter(e->r_key, e->r_off, e->r_cursor, e->r_perm, N, st);
     launch_bucket_sort(e->r_off, e->r_perm, nc3, st);
+    launch_invert(e->r_perm, e->r_iperm, N, st);
     launch_cell_gather(e->r_perm, e->pos, e->r_pos, N, st);
     const double L = e->prm.box_size, rc = e->prm.rc;
     const double cut = rc > 2.0 ? rc : 2.0;
@@ -524,8 +529,11 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
     }
   }
   const Tiles tl = resolve_tiles(e, e->tiles, C);
+  // one pass writes the forces in both sorted orders when both are needed
+  const bool both = (parts & SPME_RECIP) && !e->dense && (parts & SPME_REAL);
   if ((parts & SPME_RECIP) && !e->dense) {
-    launch_permute_in<T>(e->perm, in, ldin, (T *)e->fs, N, C, st);
+    launch_permute_in<T>(e->iperm, (T *)e->fs, both ? e->r_iperm : nullptr,
+                         both ? (T *)e->r_fs : nullptr, in, ldin, N, C, st);
     mark(SPME_ST_PERMUTE);
     SpreadArgs<T> sa{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, C, tl.spread_cc,
                      e->P, (T *)e->grid, e->comp_stride, e->RS, e->K, tl.spread_zc};
@@ -551,8 +559,10 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
     mark(SPME_ST_INTERP);
   }
   if (parts & SPME_REAL) {
-    launch_permute_in<T>(e->r_perm, in, ldin, (T *)e->r_fs, N, C, st);
-    mark(SPME_ST_PERMUTE_REAL);
+    if (!both) {
+      launch_permute_in<T>(e->r_iperm, (T *)e->r_fs, nullptr, nullptr, in, ldin, N, C, st);
+      mark(SPME_ST_PERMUTE_REAL);
+    }
     const double xi = e->prm.xi;
     const double self = (parts & SPME_SELF)
                             ? 1.0 - 6.0 * xi / std::sqrt(M_PI) +

@@ -125,12 +125,40 @@ __global__ __launch_bounds__(256) void k_pgather(const int *__restrict__ perm,
   for (int j = 0; j < P3; ++j) pw[(size_t)s * P3 + j] = T(w_tmp[(size_t)i * P3 + j]);
 }
 
+// Forces into sorted particle-major order, one or two orders in one pass:
+// fa[ia[i]*C + 3r + d] = in[r*ldin + 3i + d] (and fb through ib).  Threads
+// run over the original index i, so each input row is read once and
+// coalesced; a block writes the whole records of its 256 particles.
 template <typename T>
-__global__ __launch_bounds__(256) void k_permute_in(const int *__restrict__ perm,
-                                                    const T *__restrict__ in, long ldin,
-                                                    T *__restrict__ fs, int N, int C) {
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < (long)N * C) permute_elem<T>(perm, in, ldin, fs, C, t);
+__global__ __launch_bounds__(256) void k_permute_in(const int *__restrict__ ia,
+                                                    T *__restrict__ fa,
+                                                    const int *__restrict__ ib,
+                                                    T *__restrict__ fb,
+                                                    const T *__restrict__ in, long ldin, int N,
+                                                    int C) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  T *pa = fa + (size_t)ia[i] * C;
+  T *pb = fb ? fb + (size_t)ib[i] * C : nullptr;
+  for (int r = 0; r < C / 3; ++r) {
+    const T *src = in + r * ldin + 3L * i;
+    const T v0 = src[0], v1 = src[1], v2 = src[2];
+    pa[3 * r] = v0;
+    pa[3 * r + 1] = v1;
+    pa[3 * r + 2] = v2;
+    if (pb) {
+      pb[3 * r] = v0;
+      pb[3 * r + 1] = v1;
+      pb[3 * r + 2] = v2;
+    }
+  }
+}
+
+// inv[perm[s]] = s
+__global__ __launch_bounds__(256) void k_invert(const int *__restrict__ perm,
+                                                int *__restrict__ inv, int N) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < N) inv[perm[s]] = s;
 }
 
 // Spread: single-system kernel (logical blocks XCD-remapped).
@@ -649,10 +677,14 @@ void launch_pgather(const int *perm, const int *base_tmp, const double *w_tmp, i
 }
 
 template <typename T>
-void launch_permute_in(const int *perm, const T *in, long ldin, T *fs, int N, int C,
-                       hipStream_t st) {
-  hipLaunchKernelGGL(k_permute_in<T>, dim3(grid1((long)N * C, 256)), dim3(256), 0, st, perm, in,
-                     ldin, fs, N, C);
+void launch_permute_in(const int *ia, T *fa, const int *ib, T *fb, const T *in, long ldin, int N,
+                       int C, hipStream_t st) {
+  hipLaunchKernelGGL(k_permute_in<T>, dim3(grid1(N, 256)), dim3(256), 0, st, ia, fa, ib, fb, in,
+                     ldin, N, C);
+}
+
+void launch_invert(const int *perm, int *inv, int N, hipStream_t st) {
+  hipLaunchKernelGGL(k_invert, dim3(grid1(N, 256)), dim3(256), 0, st, perm, inv, N);
 }
 
 template <typename T, int P, int CC>
@@ -846,10 +878,10 @@ template void launch_pgather<float>(const int *, const int *, const double *, in
                                     int, int, hipStream_t);
 template void launch_pgather<double>(const int *, const int *, const double *, int *, double *,
                                      int, int, hipStream_t);
-template void launch_permute_in<float>(const int *, const float *, long, float *, int, int,
-                                       hipStream_t);
-template void launch_permute_in<double>(const int *, const double *, long, double *, int, int,
-                                        hipStream_t);
+template void launch_permute_in<float>(const int *, float *, const int *, float *, const float *,
+                                       long, int, int, hipStream_t);
+template void launch_permute_in<double>(const int *, double *, const int *, double *,
+                                        const double *, long, int, int, hipStream_t);
 template int launch_spread<float>(const SpreadArgs<float> &, hipStream_t);
 template int launch_spread<double>(const SpreadArgs<double> &, hipStream_t);
 template int launch_interp<float>(const InterpArgs<float> &, hipStream_t);

@@ -123,8 +123,9 @@ template <typename T>
 void launch_pgather(const int *perm, const int *base_tmp, const double *w_tmp, int *pbase,
                     T *pw, int N, int P3, hipStream_t st);
 template <typename T>
-void launch_permute_in(const int *perm, const T *in, long ldin, T *fs, int N, int C,
-                       hipStream_t st);
+void launch_permute_in(const int *ia, T *fa, const int *ib, T *fb, const T *in, long ldin, int N,
+                       int C, hipStream_t st);
+void launch_invert(const int *perm, int *inv, int N, hipStream_t st);
 template <typename T> int launch_spread(const SpreadArgs<T> &a, hipStream_t st);
 template <typename T> int launch_interp(const InterpArgs<T> &a, hipStream_t st);
 // mk (optional): events to record after each launch (mk->n = 0 on entry).
