@@ -128,7 +128,8 @@ def algorithmic_bytes(N, K, p, R, s, real_entries):
     bP = 3 * p * s + 12
     vec = 3 * N * R * s
     return {
-        "permute": 2 * vec + 4 * N,
+        # input read once, written in both sorted orders (the full operator)
+        "permute": 3 * vec + 8 * N,
         "spread": vec + N * bP + K3 * C * s,
         # fused x + y passes: real mesh in, half spectrum out (and back)
         "fft_x_fwd": K3 * C * s + Kh * C * 2 * s,

@@ -171,6 +171,26 @@ def test_recip_fp64_and_fp32(oracle, E, torch, K, p):
         eng.close()
 
 
+@pytest.mark.parametrize("K,p,N", [(32, 4, 6000), (32, 6, 8000), (48, 8, 15000)])
+def test_recip_dense_systems(oracle, E, torch, K, p, N):
+    """0.18-0.24 particles per mesh cell: the spreading's staging groups and
+    single steps exceed one batch, so steps are staged one by one in
+    several batches (the path the sparse workloads rarely take)."""
+    _say(f"case: recip dense K={K} p={p} N={N} fp64+fp32, 4 RHS")
+    pos, L = _system(7 + p, N, 0.2)
+    xi = 0.5
+    f = W.forces(7 + p, N, 4)
+    ref = oracle.compute_spme(xi, K, p, L, pos, f)
+    for prec, tol, dt in [(E.SPME_FP64, FP64_TOL, torch.float64),
+                          (E.SPME_FP32, FP32_TOL, torch.float32)]:
+        eng = E.Engine(xi, K, p, N, L, precision=prec, max_rhs=4)
+        eng.set_positions(torch.tensor(pos, device="cuda"))
+        u = eng.apply(torch.tensor(f, device="cuda", dtype=dt))
+        torch.cuda.synchronize()
+        assert _rel(u.double().cpu().numpy(), ref) < tol, prec
+        eng.close()
+
+
 def test_batched_rhs_equal_single(oracle, E, torch):
     _say("case: batched R=16 fp32 and R=4 fp64, K=48 p=8 N=2000")
     N, K, p = 2000, 48, 8
