@@ -110,7 +110,8 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
 
   // copy particles [b0, b0 + nb) of the concatenated sorted ranges
   // (lo[t], cnt[t]), t < NTL, into buffer B: base, NW weights and the CC
-  // forces of this pass, one dword per lane per load
+  // forces of this pass, one dword per lane per load (32-bit dword offsets:
+  // the engine bounds N so that N * C values fit)
   auto stage = [&](auto bc, const int (&lo)[NTL], const int (&cnt)[NTL], int b0, int nb) {
     constexpr int B = decltype(bc)::value;
     int *spb = B ? s_pb1 : s_pb0;

@@ -234,6 +234,9 @@ int validate(const spme_params_t *p) {
       p->precision != SPME_FP64_PLAIN)
     return fail(-1, "precision must be SPME_FP64, SPME_FP32 or SPME_FP64_PLAIN");
   if (p->max_rhs < 1 || p->max_rhs > 16) return fail(-1, "max_rhs must be in [1, 16]");
+  // the spreading stages particle data by 32-bit dword offsets into the
+  // sorted force and weight arrays (at most N * 48 values of 2 dwords)
+  if (p->np > (1 << 23)) return fail(-1, "np must be at most %d (got %d)", 1 << 23, p->np);
   return 0;
 }
 

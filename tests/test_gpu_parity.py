@@ -118,6 +118,36 @@ def test_fft_pipeline_large_meshes(oracle, E, torch, prec, K):
     assert _rel(m2, back) < (FP64_TOL if prec == "fp64" else FP32_TOL)
 
 
+@pytest.mark.parametrize("prec,K", [("fp64", 128), ("fp32", 192), ("fp32", 256)])
+def test_fft_pipeline_vs_rocfft(oracle, E, torch, prec, K):
+    """The same pipeline against rocFFT (torch.fft on the GPU) in the engine's
+    precision: r2c over the whole mesh, the oracle's influence, then c2c
+    along z and y and a c2r along x whose k_x = 0 and K/2 terms are made real
+    first (the engine's convention, and a well-defined c2r input)."""
+    _say(f"case: fft pipeline vs rocFFT {prec} K={K} N=2000 p=6")
+    N, p = 2000, 6
+    pos, L = _system(400 + K, N, 0.2)
+    xi = 0.45
+    f = W.forces(400 + K, N)[0]
+    P = E.SPME_FP64 if prec == "fp64" else E.SPME_FP32
+    dt = torch.float64 if prec == "fp64" else torch.float32
+    eng = E.Engine(xi, K, p, N, L, precision=P)
+    eng.set_positions(pos)
+    ft = torch.tensor(f, device="cuda", dtype=dt).view(1, -1)
+    m1 = eng.debug_stage(1, ft)[:, :, :, :K]
+    m2 = eng.debug_stage(2, ft)[:, :, :, :K]
+    eng.close()
+    lm2 = oracle.compute_lm2(xi, K, p, L)
+    spec = torch.fft.rfftn(torch.tensor(np.ascontiguousarray(m1), device="cuda"), dim=(1, 2, 3))
+    spec = oracle.apply_influence(K, L, lm2, spec.cpu().numpy().astype(np.complex128))
+    g = torch.tensor(spec, device="cuda").to(torch.complex128 if prec == "fp64" else torch.complex64)
+    g = torch.fft.ifftn(g, dim=(1, 2))
+    g[..., 0] = g[..., 0].real.to(g.dtype)
+    g[..., K // 2] = g[..., K // 2].real.to(g.dtype)
+    back = torch.fft.irfft(g, n=K, dim=3).double().cpu().numpy() * (K ** 3 / L ** 3)
+    assert _rel(m2, back) < (FP64_TOL if prec == "fp64" else FP32_TOL)
+
+
 def test_c2_fp64_fp32_vs_oracle_and_direct(oracle, E, torch):
     """The plain-vs-FP32 workload: N = 100,000, phi = 0.2, K = 128, p = 6,
     rc = 4, one right-hand side, the same xi (the one minimising the error
