@@ -433,6 +433,16 @@ def main():
                          f"(P build included), {tc:.1f} s"}
         print(f"[bench] cpu: {tc:.2f}s for {nr} M.f on {O.num_threads()} threads",
               file=sys.stderr, flush=True)
+        # p = 8 is beyond the reference (old-version/spme.c:86 asserts p in
+        # {4, 6}): the same workload at its largest order, same K, xi, rc
+        t0 = time.perf_counter()
+        O.spme_full(xi, K, 6, rc, L, sysm.pos, f64[:nr])
+        tc6 = time.perf_counter() - t0
+        cpu["reference_order_p6"] = {
+            "value": round(nr / tc6, 4), "unit": "M.f/s",
+            "sample": f"{nr} right-hand side(s), same workload at p=6 (B-spline tables equal "
+                      f"the reference's literals), {tc6:.1f} s"}
+        print(f"[bench] cpu p=6: {tc6:.2f}s for {nr} M.f", file=sys.stderr, flush=True)
 
     out = {
         "metric": "RPY M.f products/sec at N=200k, K=192^3 (FP32 batched vs FP64 plain); % HBM",
