@@ -41,7 +41,7 @@ typedef struct spme_params {
   double xi;       /* Ewald splitting parameter */
   int dim;         /* mesh points per side, K */
   int porder;      /* B-spline order p: 4, 6 or 8 */
-  int np;          /* number of particles N */
+  int np;          /* number of particles N, at most 2^23 */
   double box_size; /* cube side L */
   double rc;       /* real-space cutoff (used by SPME_REAL) */
   int precision;   /* SPME_FP64, SPME_FP32 or SPME_FP64_PLAIN (see above) */
@@ -71,7 +71,8 @@ int spme_engine_apply(spme_t *spme, int nrhs, double alpha, const void *in, long
  * per stage, the summed milliseconds and the number of launches since
  * timing was (re)enabled.  Stages, in launch order: */
 enum {
-  SPME_ST_PERMUTE = 0,   /* forces into sorted order (reciprocal part) */
+  SPME_ST_PERMUTE = 0,   /* forces into sorted order (reciprocal part; with the real
+                            part too, both orders in this one launch) */
   SPME_ST_SPREAD,        /* B-spline spreading onto the mesh */
   SPME_ST_FFT_X_FWD,     /* r2c along x */
   SPME_ST_FFT_Y_FWD,     /* complex FFT along y */
@@ -79,7 +80,8 @@ enum {
   SPME_ST_FFT_Y_INV,     /* inverse FFT along y */
   SPME_ST_FFT_X_INV,     /* c2r along x */
   SPME_ST_INTERP,        /* interpolation back to particles */
-  SPME_ST_PERMUTE_REAL,  /* forces into cell order (real-space part) */
+  SPME_ST_PERMUTE_REAL,  /* forces into cell order (real-space part without the
+                            reciprocal part) */
   SPME_ST_REAL,          /* real-space pairs, overlap and self */
   SPME_NSTAGES
 };
