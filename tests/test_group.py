@@ -147,10 +147,13 @@ def test_group_tiles_and_autotune_are_bitwise_neutral():
     g.close()
 
 
-def test_engine_tiles_and_autotune_are_bitwise_neutral():
+@pytest.mark.parametrize("n,K", [(6000, 64), (20000, 128)])
+def test_engine_tiles_and_autotune_are_bitwise_neutral(n, K):
+    """Every tile shape gives the default's bits; at K = 128 a z chunk of 256
+    (clamped to 128) is spread in two sweeps (96 + 32 planes)."""
     torch = _torch()
     from spme_amd import engine as E
-    m = W.GroupMember(5151, 6000, 0.2, W.box_length(6000, 0.2), 64, 8, 0.5, 6.0)
+    m = W.GroupMember(5151 + K, n, 0.2, W.box_length(n, 0.2), K, 8, 0.5, 6.0)
     for prec, dt in ((E.SPME_FP32, torch.float32), (E.SPME_FP64, torch.float64)):
         e = E.Engine(m.xi, m.K, m.p, m.n, m.L, rc=m.rc, precision=prec, max_rhs=NRHS)
         e.set_positions(W.positions(m.seed, m.n, m.L))
