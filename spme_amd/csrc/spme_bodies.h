@@ -850,11 +850,14 @@ __device__ __forceinline__ void fft_y_block(T *__restrict__ grid, long comp_stri
 // Wave numbers follow old-version/spme.c:95-107 (index K/2 -> +pi K/L).
 // The spectrum of a (column, k1) stays in registers between the forward
 // N2-point FFT, the 3x3 influence and the backward N2-point FFT.
+// nyq: the Nyquist column as [component][z][y] (its blocks then read 16
+// consecutive y per z), or null when it sits in the spectrum rows.
 template <typename T, int K, int XW, int NT>
 __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp_stride,
                                                  int Khp, const T *__restrict__ tab, T dk,
                                                  const cpx<T> *__restrict__ tw, int lb,
-                                                 cpx<T> *buf, cpx<T> *s_tw) {
+                                                 cpx<T> *buf, cpx<T> *s_tw,
+                                                 cpx<T> *__restrict__ nyq) {
   constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
   constexpr int Kh = K / 2 + 1;
   constexpr int LS = K + 1;
@@ -873,6 +876,9 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
   }
   cpx<T> *g0 = reinterpret_cast<cpx<T> *>(grid + (3L * rhs) * comp_stride);
   const long cs = comp_stride / 2;  // complex elements
+  // this block's column values in the separate Nyquist array (block-uniform)
+  cpx<T> *nq = (nyq && j >= NXC * K) ? nyq + 3L * rhs * K * K : nullptr;
+  constexpr long NQC = (long)K * K;  // Nyquist values per component
   for (int t = threadIdx.x; t < 3 * XW * N2; t += NT) {  // step A (forward) from global
     const int xo = t % XW, r = t / XW;
     const int c = r % 3, n2 = r / 3;
@@ -880,8 +886,9 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
     cpx<T> a[N1];
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1)
-      a[n1] = x < Kh ? g0[c * cs + ((size_t)(n1 * N2 + n2) * K + y) * Khp + x]
-                     : cpx<T>{T(0), T(0)};
+      a[n1] = nq ? nq[c * NQC + (long)(n1 * N2 + n2) * K + y]
+                 : (x < Kh ? g0[c * cs + ((size_t)(n1 * N2 + n2) * K + y) * Khp + x]
+                           : cpx<T>{T(0), T(0)});
     reg_fft<N1, -1>(a);
 #pragma unroll
     for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<-1>(s_tw, n2 * k1));
@@ -952,7 +959,10 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
 #pragma unroll
     for (int k1 = 0; k1 < N1; ++k1) a[k1] = buf[(c * XW + xo) * LS + k1 * N2 + n2];
     reg_fft<N1, +1>(a);
-    if (x < Kh) {
+    if (nq) {
+#pragma unroll
+      for (int n1 = 0; n1 < N1; ++n1) nq[c * NQC + (long)(n2 + N2 * n1) * K + y] = a[n1];
+    } else if (x < Kh) {
 #pragma unroll
       for (int n1 = 0; n1 < N1; ++n1)
         g0[c * cs + ((size_t)(n2 + N2 * n1) * K + y) * Khp + x] = a[n1];

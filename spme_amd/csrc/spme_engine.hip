@@ -141,6 +141,7 @@ struct _spme_t {
   bool have_pos;
   // mesh and tables
   void *grid;  // 3*max_rhs components
+  void *nyq;   // Nyquist column of the fused FFT path, [3*max_rhs][K][K] complex (or null)
   void *tab;   // plain path: influence table [K][K][Khp]; otherwise its factors
   void *tw;    // twiddles exp(-2 pi i t/K), K complex
   // P data
@@ -241,7 +242,7 @@ int validate(const spme_params_t *p) {
 }
 
 void free_all(_spme_t *e) {
-  void *ptrs[] = {e->grid, e->tab, e->tw, e->pos, e->base_tmp, e->key_tmp, e->counts,
+  void *ptrs[] = {e->grid, e->nyq, e->tab, e->tw, e->pos, e->base_tmp, e->key_tmp, e->counts,
                   e->cursor, e->boff, e->perm, e->pbase, e->err, e->w_tmp, e->pw, e->fs,
                   e->r_key, e->r_counts, e->r_cursor, e->r_off, e->r_perm, e->r_pos,
                   e->r_fs, e->r_cnt, e->r_width, e->r_size, e->r_soff, e->r_col, e->r_blk,
@@ -290,6 +291,8 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
   E_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   // the plain path transforms one right-hand side (3 components) at a time
   E_TRY(hipMalloc(&e->grid, (size_t)(e->dense ? 3 : C) * e->comp_stride * e->tsz));
+  const bool fused = e->prec == SPME_FP32 ? fft_fused<float>(K) : fft_fused<double>(K);
+  if (!e->dense && fused) E_TRY(hipMalloc(&e->nyq, (size_t)C * K * K * 2 * e->tsz));
   int rc = e->prec == SPME_FP32 ? upload_tables<float>(e) : upload_tables<double>(e);
   if (rc) return bail(rc);
   e->nkeys = (K / 8) * (K / 8) * K;
@@ -544,7 +547,8 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
     mark(SPME_ST_SPREAD);
     if (stop_after == 1) return 0;
     FftArgs<T> fa{(T *)e->grid, e->comp_stride, e->RS, e->K, C, (const T *)e->tab,
-                  T(2.0 * M_PI / e->prm.box_size), (const cpx<T> *)e->tw, e->ncu};
+                  T(2.0 * M_PI / e->prm.box_size), (const cpx<T> *)e->tw, e->ncu,
+                  (cpx<T> *)e->nyq};
     FftMarks fm{};
     if (timed)
       for (int i = 0; i < 5; ++i) fm.ev[i] = take_event(e);

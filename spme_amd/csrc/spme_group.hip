@@ -186,7 +186,7 @@ __global__ __launch_bounds__(NT) void kg_fft_z_infl(GroupStage<FftArgs<T>> t) {
   case KK:                                                                                   \
     if constexpr (KK <= KMAX)                                                                \
       fft_z_infl_block<T, KK, XW, NT>(a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw, lb, buf, \
-                                      s_tw);                                                 \
+                                      s_tw, nullptr);                                        \
     break;
       SPME_FOR_EACH_K(X)
 #undef X

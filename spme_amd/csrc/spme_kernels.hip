@@ -227,11 +227,12 @@ __global__ __launch_bounds__(NT) void k_fft_z(T *__restrict__ grid, long comp_st
 template <typename T, int K, int XW, int NT>
 __global__ __launch_bounds__(NT) void k_fft_z_infl(T *__restrict__ grid, long comp_stride,
                                                    int Khp, const T *__restrict__ tab, T dk,
-                                                   const cpx<T> *__restrict__ tw) {
+                                                   const cpx<T> *__restrict__ tw,
+                                                   cpx<T> *__restrict__ nyq) {
   __shared__ cpx<T> buf[3 * XW * (K + 1)];
   __shared__ cpx<T> s_tw[K];
   fft_z_infl_block<T, K, XW, NT>(grid, comp_stride, Khp, tab, dk, tw,
-                                 xcd_remap(blockIdx.x, gridDim.x), buf, s_tw);
+                                 xcd_remap(blockIdx.x, gridDim.x), buf, s_tw, nyq);
 }
 
 // ---------------------------------------------------------------------------
@@ -283,7 +284,8 @@ __device__ __forceinline__ void xy_pack(cpx<T> *line, int k) {
 
 template <typename T, int K, int NT>
 __global__ __launch_bounds__(NT) void k_fft_xy_fwd(T *__restrict__ grid, int RS, long comp_stride,
-                                                   int nplanes, const cpx<T> *__restrict__ tw) {
+                                                   int nplanes, const cpx<T> *__restrict__ tw,
+                                                   cpx<T> *__restrict__ nyq) {
   using S = XYShape<T, K>;
   constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
   constexpr int Kh = S::Kh, LP = S::LP, L2 = K / 2;
@@ -399,14 +401,21 @@ __global__ __launch_bounds__(NT) void k_fft_xy_fwd(T *__restrict__ grid, int RS,
     {
       const int comp = q / K, z = q - comp * K;
       cpx<T> *spec = reinterpret_cast<cpx<T> *>(grid + comp * comp_stride) + (size_t)z * K * Khp;
+      // the Nyquist column goes to its own array when there is one
+      cpx<T> *nq = nyq ? nyq + ((size_t)comp * K + z) * K : nullptr;
       for (int t = tid; t < Kh * N1; t += NT) {
         const int x = t % Kh, k1 = t / Kh;
         cpx<T> bb[N2];
 #pragma unroll
         for (int n2 = 0; n2 < N2; ++n2) bb[n2] = buf[(k1 * N2 + n2) * Kh + x];
         reg_fft<N2, -1>(bb);
+        if (nq && x == K / 2) {
 #pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) spec[(size_t)(k1 + N1 * k2) * Khp + x] = bb[k2];
+          for (int k2 = 0; k2 < N2; ++k2) nq[k1 + N1 * k2] = bb[k2];
+        } else {
+#pragma unroll
+          for (int k2 = 0; k2 < N2; ++k2) spec[(size_t)(k1 + N1 * k2) * Khp + x] = bb[k2];
+        }
       }
     }
     __syncthreads();
@@ -415,12 +424,19 @@ __global__ __launch_bounds__(NT) void k_fft_xy_fwd(T *__restrict__ grid, int RS,
 
 template <typename T, int K, int NT>
 __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS, long comp_stride,
-                                                   int nplanes, const cpx<T> *__restrict__ tw) {
+                                                   int nplanes, const cpx<T> *__restrict__ tw,
+                                                   const cpx<T> *__restrict__ nyq) {
   using S = XYShape<T, K>;
   constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
   constexpr int Kh = S::Kh, LP = S::LP, L2 = K / 2;
   constexpr int G = 16 / sizeof(cpx<T>);  // complex per 16-byte load
-  constexpr int UPR = (Kh + G - 1) / G, NUNIT = K * UPR, NU = (NUNIT + NT - 1) / NT;
+  // units of G complex per plane: whole rows x < Kh (UPR per row), or, with
+  // a separate Nyquist column, rows x < K/2 (UPQ per row) and the column
+  // (K/G units)
+  constexpr int UPR = (Kh + G - 1) / G, UPQ = (K / 2) / G;
+  static_assert((K / 2) % G == 0 && K % G == 0, "whole units");
+  constexpr int NUNIT = K * UPR > K * UPQ + K / G ? K * UPR : K * UPQ + K / G;
+  constexpr int NU = (NUNIT + NT - 1) / NT;
   __shared__ cpx<T> buf[S::NCPX];
   __shared__ cpx<T> s_tw[K];
   struct alignas(16) C16 { cpx<T> v[G]; };
@@ -428,16 +444,20 @@ __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS,
   int tid = threadIdx.x;
   const int Khp = RS / 2;
   C16 rv[NU];
+  const int upr = nyq ? UPQ : UPR;               // row units
+  const int nunit = nyq ? K * UPQ + K / G : K * UPR;
   auto fetch = [&](int q) {
     const int comp = q / K, z = q - comp * K;
     const cpx<T> *spec =
         reinterpret_cast<const cpx<T> *>(grid + comp * comp_stride) + (size_t)z * K * Khp;
+    const cpx<T> *nq = nyq ? nyq + ((size_t)comp * K + z) * K : nullptr;
 #pragma unroll
     for (int r = 0; r < NU; ++r) {
       const int u = tid + NT * r;
-      if (u < NUNIT) {
-        const int y = u / UPR, g = u - y * UPR;
-        rv[r] = *reinterpret_cast<const C16 *>(spec + (size_t)y * Khp + G * g);
+      if (u < nunit) {
+        const int y = u / upr, g = u - y * upr;
+        rv[r] = y < K ? *reinterpret_cast<const C16 *>(spec + (size_t)y * Khp + G * g)
+                      : *reinterpret_cast<const C16 *>(nq + G * (u - K * upr));
       }
     }
   };
@@ -445,11 +465,17 @@ __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS,
 #pragma unroll
     for (int r = 0; r < NU; ++r) {
       const int u = tid + NT * r;
-      if (u < NUNIT) {
-        const int y = u / UPR, g = u - y * UPR;
+      if (u < nunit) {
+        const int y = u / upr, g = u - y * upr;
+        if (y < K) {
 #pragma unroll
-        for (int i = 0; i < G; ++i)
-          if (G * g + i < Kh) buf[y * Kh + G * g + i] = rv[r].v[i];
+          for (int i = 0; i < G; ++i)
+            if (G * g + i < Kh) buf[y * Kh + G * g + i] = rv[r].v[i];
+        } else {  // Nyquist units: rows G w .. G w + G - 1 of column K/2
+          const int w = u - K * upr;
+#pragma unroll
+          for (int i = 0; i < G; ++i) buf[(G * w + i) * Kh + K / 2] = rv[r].v[i];
+        }
       }
     }
   };
@@ -760,6 +786,17 @@ template <typename T, int K> constexpr int xy_threads() {
   return K >= 192 ? 768 : (K >= 128 ? (sizeof(T) == 8 ? 768 : 512) : 256);
 }
 
+template <typename T> bool fft_fused(int K) {
+  switch (K) {
+    case 128: return xy_fits<T, 128>();
+    case 192: return xy_fits<T, 192>();
+    case 256: return xy_fits<T, 256>();
+    default: return false;  // xy_fits needs K >= 128
+  }
+}
+template bool fft_fused<float>(int);
+template bool fft_fused<double>(int);
+
 template <typename T, int K>
 static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
   using S = FftShape<T>;
@@ -782,13 +819,13 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
     const int per_cu = std::max(1, std::min(2, int((160 * 1024) / XYShape<T, K>::LDS)));
     const int grid = std::min(nplanes, std::max(1, a.ncu) * per_cu);
     hipLaunchKernelGGL((k_fft_xy_fwd<T, K, NTXY>), dim3(grid), dim3(NTXY), 0, st, a.grid, a.RS,
-                       a.comp_stride, nplanes, a.tw);
+                       a.comp_stride, nplanes, a.tw, a.nyq);
     mark(0);
     hipLaunchKernelGGL((k_fft_z_infl<T, K, ZS::XW, ZS::NT>), dim3(nzb), dim3(ZS::NT), 0, st,
-                       a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw);
+                       a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw, a.nyq);
     mark(2);
     hipLaunchKernelGGL((k_fft_xy_inv<T, K, NTXY>), dim3(grid), dim3(NTXY), 0, st, a.grid, a.RS,
-                       a.comp_stride, nplanes, a.tw);
+                       a.comp_stride, nplanes, a.tw, a.nyq);
     mark(4);
   } else {
     hipLaunchKernelGGL((k_fft_x_r2c<T, K, S::NLX, NT>), dim3(grid1(npairs, S::NLX)), dim3(NT),
@@ -798,7 +835,7 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
                        st, a.grid, a.comp_stride, Khp, nxc, a.tw);
     mark(1);
     hipLaunchKernelGGL((k_fft_z_infl<T, K, ZS::XW, ZS::NT>), dim3(nzb), dim3(ZS::NT), 0, st,
-                       a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw);
+                       a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw, nullptr);
     mark(2);
     hipLaunchKernelGGL((k_fft_y<T, K, S::XW, +1, NT>), dim3(nxc * K * a.ncomp), dim3(NT), 0,
                        st, a.grid, a.comp_stride, Khp, nxc, a.tw);

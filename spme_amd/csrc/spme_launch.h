@@ -101,7 +101,14 @@ template <typename T> struct FftArgs {
   T dk;
   const cpx<T> *tw;
   int ncu;  // compute units (grid of the fused plane passes)
+  // Nyquist column x = K/2 of the half spectrum as [component][z][y]
+  // complex, used by the fused plane passes and the z pass; null: the column
+  // stays in the spectrum rows (separate passes, grouped applies)
+  cpx<T> *nyq;
 };
+// True when the single-system apply of this mesh uses the fused x + y plane
+// passes (and then a separate Nyquist column).
+template <typename T> bool fft_fused(int K);
 
 // Events recorded after each FFT launch and the pass each one ends
 // (0 x or fused x+y forward, 1 y forward, 2 z + influence, 3 y inverse,
