@@ -514,7 +514,7 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
     double *uout = reinterpret_cast<double *>(out);
     double *grid = (double *)e->grid;
     FftArgs<double> fa{grid, cs, e->RS, K, 3, (const double *)e->tab, 2.0 * M_PI / L,
-                       (const cpx<double> *)e->tw, e->ncu};
+                       (const cpx<double> *)e->tw, e->ncu, nullptr};
     const double k3 = double(K) * K * K;
     for (int r = 0; r < nrhs; ++r) {
       HIP_TRY(hipMemsetAsync(grid, 0, sizeof(double) * 3 * cs, st));
@@ -851,8 +851,9 @@ int group_build_args(_spme_group_t *g, double alpha, const void *const *in, cons
     sa[s] = SpreadArgs<T>{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, C,
                           tl.spread_cc, e->P, (T *)e->grid, e->comp_stride, e->RS, e->K,
                           tl.spread_zc};
+    // grouped applies run the separate passes: the Nyquist column stays in the rows
     fa[s] = FftArgs<T>{(T *)e->grid, e->comp_stride, e->RS, e->K, C, (const T *)e->tab,
-                       T(2.0 * M_PI / e->prm.box_size), (const cpx<T> *)e->tw};
+                       T(2.0 * M_PI / e->prm.box_size), (const cpx<T> *)e->tw, e->ncu, nullptr};
     ia[s] = InterpArgs<T>{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid,
                           e->comp_stride, e->RS, e->K, e->N, C, e->P, tl.interp_zc, T(alpha),
                           T(beta), (T *)out[s], ldout[s]};
