@@ -126,9 +126,15 @@ __global__ __launch_bounds__(256) void k_pgather(const int *__restrict__ perm,
 }
 
 // Forces into sorted particle-major order, one or two orders in one pass:
-// fa[ia[i]*C + 3r + d] = in[r*ldin + 3i + d] (and fb through ib).  Threads
-// run over the original index i, so each input row is read once and
-// coalesced; a block writes the whole records of its 256 particles.
+// fa[ia[i]*C + 3r + d] = in[r*ldin + 3i + d] (and fb through ib), C <= 48.
+// A block stages PB consecutive original particles in LDS (each input row
+// read once, coalesced), then writes each particle's C values as one
+// contiguous record per order.
+template <typename T> struct PermuteShape {
+  static constexpr int PB = sizeof(T) == 8 ? 128 : 256;  // particles per block
+  static constexpr int CP = 48 + 1;                       // LDS pitch per particle
+};
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_permute_in(const int *__restrict__ ia,
                                                     T *__restrict__ fa,
@@ -136,21 +142,22 @@ __global__ __launch_bounds__(256) void k_permute_in(const int *__restrict__ ia,
                                                     T *__restrict__ fb,
                                                     const T *__restrict__ in, long ldin, int N,
                                                     int C) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  T *pa = fa + (size_t)ia[i] * C;
-  T *pb = fb ? fb + (size_t)ib[i] * C : nullptr;
+  constexpr int PB = PermuteShape<T>::PB, CP = PermuteShape<T>::CP;
+  __shared__ T s[PB * CP];
+  const int i0 = blockIdx.x * PB, n = min(PB, N - i0), tid = threadIdx.x;
   for (int r = 0; r < C / 3; ++r) {
-    const T *src = in + r * ldin + 3L * i;
-    const T v0 = src[0], v1 = src[1], v2 = src[2];
-    pa[3 * r] = v0;
-    pa[3 * r + 1] = v1;
-    pa[3 * r + 2] = v2;
-    if (pb) {
-      pb[3 * r] = v0;
-      pb[3 * r + 1] = v1;
-      pb[3 * r + 2] = v2;
+    const T *src = in + r * ldin + 3L * i0;
+    for (int e = tid; e < 3 * n; e += 256) {
+      const int i = e / 3;
+      s[i * CP + 3 * r + (e - 3 * i)] = src[e];
     }
+  }
+  __syncthreads();
+  for (int e = tid; e < n * C; e += 256) {
+    const int i = e / C, c = e - i * C;
+    const T v = s[i * CP + c];
+    fa[(size_t)ia[i0 + i] * C + c] = v;
+    if (fb) fb[(size_t)ib[i0 + i] * C + c] = v;
   }
 }
 
@@ -705,8 +712,8 @@ void launch_pgather(const int *perm, const int *base_tmp, const double *w_tmp, i
 template <typename T>
 void launch_permute_in(const int *ia, T *fa, const int *ib, T *fb, const T *in, long ldin, int N,
                        int C, hipStream_t st) {
-  hipLaunchKernelGGL(k_permute_in<T>, dim3(grid1(N, 256)), dim3(256), 0, st, ia, fa, ib, fb, in,
-                     ldin, N, C);
+  hipLaunchKernelGGL(k_permute_in<T>, dim3(grid1(N, PermuteShape<T>::PB)), dim3(256), 0, st, ia,
+                     fa, ib, fb, in, ldin, N, C);
 }
 
 void launch_invert(const int *perm, int *inv, int N, hipStream_t st) {
