@@ -324,3 +324,43 @@ def test_spme_full_converges_to_direct(oracle):
         e.append(np.linalg.norm(u - ref) / np.linalg.norm(ref))
     assert e[0] > e[1] > e[2]
     assert e[2] < 1e-7
+
+
+def _asymmetry(f, u):
+    """max |f_a . M f_b - f_b . M f_a| / (|f| |M f|) over pairs (a, b)."""
+    G = f @ u.T
+    scale = np.outer(np.linalg.norm(f, axis=1), np.linalg.norm(u, axis=1))
+    return float((np.abs(G - G.T) / np.maximum(scale, scale.T)).max())
+
+
+def test_symmetry_check_catches_planted_mistakes(oracle):
+    """The symmetry bar the GPU applies at the full batched size (1e-6 of
+    |f| |M f|) is met by the reciprocal operator composed from the oracle's
+    stages, and missed by orders of magnitude when a mistake is planted in
+    the composition: the interpolation stencil one mesh cell off, or two
+    force components exchanged between spreading and interpolation."""
+    N, K, p, xi = 300, 32, 6, 0.5
+    L = W.box_length(N, 0.1)
+    pos = W.positions(41, N, L)
+    f = W.forces(41, N, 6)
+    P, ind, _, _ = oracle.compute_P(K, p, pos, L)
+    lm2 = oracle.compute_lm2(xi, K, p, L)
+
+    def recip(fv, plant=None):
+        grid = oracle.spread(K, p, P, ind, fv)
+        spec = oracle.apply_influence(K, L, lm2,
+                                      np.stack([oracle.fft3d_r2c(grid[c]) for c in range(3)]))
+        g = np.stack([oracle.fft3d_c2r(spec[c]) for c in range(3)]) / K ** 3
+        if plant == "stencil":
+            g = np.roll(g, 1, axis=1)
+        elif plant == "components":
+            g = g[[1, 0, 2]]
+        return oracle.interpolate(K, p, P, ind, g, K ** 3 / L ** 3, 0.0, np.zeros(3 * N))
+
+    u = np.stack([recip(fv) for fv in f])
+    np.testing.assert_allclose(u, oracle.compute_spme(xi, K, p, L, pos, f),
+                               rtol=0, atol=1e-12 * np.abs(u).max())
+    assert _asymmetry(f, u) < 1e-12
+    for plant in ("stencil", "components"):
+        bad = np.stack([recip(fv, plant) for fv in f])
+        assert _asymmetry(f, bad) > 1e-4, plant
