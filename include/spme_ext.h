@@ -169,12 +169,14 @@ int spme_mlp_adam(float *w, const float *g, float *m1, float *m2, float lr, floa
  * right-hand sides (block Lanczos for M^{1/2} z in Brownian dynamics).  No
  * counterpart in the reference.  Blocks are row-major [s][ld] device arrays
  * of the precision's type (float for SPME_FP32, double otherwise) with
- * 1 <= s <= 16 and ld >= n; small matrices are FP64 row-major device arrays.
+ * ld >= n, 1 <= sy <= 16 rows of Y and 1 <= sx <= 4096 rows of X (X may
+ * stack every earlier block of a Krylov basis: one call instead of one per
+ * block); small matrices are FP64 row-major device arrays.
  *   gram:   C[a][b] = sum_{i<n} X[a][i] Y[b][i]   (a < sx, b < sy; FP64
  *           accumulation, deterministic; work: spme_block_gram_worksize
- *           doubles of device memory)
+ *           doubles of device memory, 0 for sizes out of range)
  *   update: Y[b][i] = beta Y[b][i] + sum_a X[a][i] A[a][b]  (beta == 0 does
- *           not read Y; X and Y must not overlap)
+ *           not read Y; the spans of X and Y must not overlap)
  * Asynchronous on stream (a hipStream_t, NULL: the default stream). */
 long spme_block_gram_worksize(long n, int sx, int sy);
 int spme_block_gram(int precision, long n, int sx, int sy, const void *X, long ldx,

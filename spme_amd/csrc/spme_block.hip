@@ -6,9 +6,11 @@
 //   gram:   C[a][b] = sum_i X[a][i] Y[b][i]        (FP64 accumulation)
 //   update: Y[b][i] = beta Y[b][i] + sum_a X[a][i] A[a][b]
 //
-// The gram is deterministic: a fixed number of blocks (a function of n
-// only) each sum a fixed set of chunks in a fixed order into a workspace of
-// partial sums, then one block per entry reduces them in a fixed tree.
+// Y has at most 16 rows; X up to 4096 (the tall form: all earlier blocks
+// of a Krylov basis stacked, one pass over them instead of one call each).
+// The gram is deterministic: a fixed number of blocks (a function of n and
+// sx only) each sum a fixed set of chunks in a fixed order into a workspace
+// of partial sums, then one block per entry reduces them in a fixed tree.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,11 +31,24 @@ int fail(int code, const char *fmt, ...) {
   return set_last_error(code, buf);
 }
 
-constexpr int BS = 16;      // largest block width
+constexpr int BS = 16;      // largest block width (Y; X up to 16: square form)
+constexpr int BX = 4096;    // most rows of X (tall form)
 constexpr int GCH = 256;    // vector entries per staged chunk
 constexpr int GMAXB = 512;  // most partial-sum blocks
+constexpr int TA = 64;      // tall form: rows of X per block
+constexpr int TCH = 64;     // tall form: vector entries per staged chunk
+constexpr int TGRID = 2048; // tall form: blocks per launch (about)
 
 int gram_blocks(long n) { return int(std::min<long>((n + GCH - 1) / GCH, GMAXB)); }
+// tall form: blocks along n per group of TA rows; each block leaves one
+// partial sum per wave (4)
+int gram_blocks_tall(long n, int sx) {
+  const int nab = (sx + TA - 1) / TA;
+  return int(std::min<long>((n + TCH - 1) / TCH, std::max(8, TGRID / nab)));
+}
+int gram_parts(long n, int sx) {
+  return sx <= BS ? gram_blocks(n) : 4 * gram_blocks_tall(n, sx);
+}
 
 // Block g sums chunks g, g + nb, g + 2 nb, ... of the vectors; thread
 // (a, b) keeps its entry's partial sum in a register across chunks.
@@ -60,6 +75,88 @@ __global__ __launch_bounds__(256) void k_block_gram_part(long n, int sx, int sy,
       for (int k = 0; k < m; ++k) s = fma(double(xs[a][k]), double(ys[b][k]), s);
   }
   if (a < sx && b < sy) part[(size_t)blockIdx.x * (sx * sy) + a * sy + b] = s;
+}
+
+// Tall form, block (g, ab): rows ab*TA .. ab*TA + TA - 1 of X against all
+// rows of Y over chunks g, g + gridDim.x, ...  Wave w sums entries
+// w*TCH/4 .. (w+1)*TCH/4 - 1 of each chunk; its lane (rg, cg) keeps the 4 x 4
+// tile rows 4rg.., columns 4cg.. in registers (4 + 4 LDS reads per 16 FMAs,
+// Y staged transposed in FP64) and leaves it as partial sum 4g + w.
+template <typename T>
+__global__ __launch_bounds__(256) void k_block_gram_tall(long n, int sx, int sy,
+                                                         const T *__restrict__ X, long ldx,
+                                                         const T *__restrict__ Y, long ldy,
+                                                         double *__restrict__ part) {
+  static_assert(TA == 64 && BS == 16 && TA * TCH % 256 == 0 && BS * TCH % 256 == 0,
+                "lane tiles: 16 x 4 of 4 x 4; 256 threads stage whole tiles");
+  constexpr int KQ = TCH / 4;  // entries per wave per chunk
+  __shared__ T xs[TA][TCH + 1];
+  __shared__ __attribute__((aligned(16))) double yt[TCH][BS];
+  const int a0 = blockIdx.y * TA, na = min(TA, sx - a0);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;  // 256 threads: w < 4
+  const int rg = lane & 15, cg = lane >> 4;                 // rows 4rg.., columns 4cg..
+  const long nch = (n + TCH - 1) / TCH;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+  // the next chunk's entries, loaded into registers while the current one
+  // is summed: thread t holds entries t + 256 q of the X tile and of Y
+  constexpr int NXR = TA * TCH / 256, NYR = BS * TCH / 256;
+  T xr[NXR], yr[NYR];
+  auto fetch = [&](long c) {
+    const long i0 = c * TCH;
+    const int m = int(n - i0 < TCH ? n - i0 : TCH);
+#pragma unroll
+    for (int q = 0; q < NXR; ++q) {
+      const int t = threadIdx.x + 256 * q, r = t / TCH, k = t - r * TCH;
+      xr[q] = (r < na && k < m) ? X[(long)(a0 + r) * ldx + i0 + k] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < NYR; ++q) {
+      const int t = threadIdx.x + 256 * q, r = t / TCH, k = t - r * TCH;
+      yr[q] = (r < sy && k < m) ? Y[(long)r * ldy + i0 + k] : T(0);
+    }
+  };
+  if (blockIdx.x < nch) fetch(blockIdx.x);
+  for (long c = blockIdx.x; c < nch; c += gridDim.x) {
+    __syncthreads();  // the previous chunk has been read
+#pragma unroll
+    for (int q = 0; q < NXR; ++q) {
+      const int t = threadIdx.x + 256 * q, r = t / TCH, k = t - r * TCH;
+      xs[r][k] = xr[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NYR; ++q) {
+      const int t = threadIdx.x + 256 * q, r = t / TCH, k = t - r * TCH;
+      yt[k][r] = double(yr[q]);
+    }
+    __syncthreads();
+    if (c + gridDim.x < nch) fetch(c + gridDim.x);
+#pragma unroll 4
+    for (int k = w * KQ; k < w * KQ + KQ; ++k) {  // entries past m are zero
+      double x[4], y[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) x[i] = double(xs[4 * rg + i][k]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) y[j] = yt[k][4 * cg + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fma(x[i], y[j], acc[i][j]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int a = 4 * rg + i;
+    if (a < na) {
+      double *p = part + ((size_t)(4 * blockIdx.x + w) * sx + a0 + a) * sy;
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (4 * cg + j < sy) p[4 * cg + j] = acc[i][j];
+    }
+  }
 }
 
 // Entry e = blockIdx.x: lane l sums partials l, l + 64, ... in order, then a
@@ -99,12 +196,62 @@ __global__ __launch_bounds__(256) void k_block_update(long n, int sx, int sy,
   }
 }
 
+// Tall form: thread i sums over all rows a of X in order (TA rows of A at
+// a time in LDS), 16 FP64 partial sums in registers.
+template <typename T>
+__global__ __launch_bounds__(256) void k_block_update_tall(long n, int sx, int sy,
+                                                           const T *__restrict__ X, long ldx,
+                                                           const double *__restrict__ A,
+                                                           double beta, T *__restrict__ Y,
+                                                           long ldy) {
+  __shared__ __attribute__((aligned(16))) double sA[TA][BS];
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool on = i < n;
+  double acc[BS];
+#pragma unroll
+  for (int b = 0; b < BS; ++b) acc[b] = 0.0;
+  for (int a0 = 0; a0 < sx; a0 += TA) {
+    const int na = min(TA, sx - a0);
+    __syncthreads();  // the previous rows of A have been read
+    for (int t = threadIdx.x; t < TA * BS; t += blockDim.x) {
+      const int r = t / BS, b = t - r * BS;
+      sA[r][b] = (r < na && b < sy) ? A[(size_t)(a0 + r) * sy + b] : 0.0;
+    }
+    __syncthreads();
+    if (on) {
+      int r = 0;
+      for (; r + 8 <= na; r += 8) {  // eight rows' loads in flight
+        double x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = double(X[(long)(a0 + r + q) * ldx + i]);
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int b = 0; b < BS; ++b) acc[b] = fma(x[q], sA[r + q][b], acc[b]);
+      }
+      for (; r < na; ++r) {
+        const double x = double(X[(long)(a0 + r) * ldx + i]);
+#pragma unroll
+        for (int b = 0; b < BS; ++b) acc[b] = fma(x, sA[r][b], acc[b]);
+      }
+    }
+  }
+  if (on) {
+#pragma unroll
+    for (int b = 0; b < BS; ++b)
+      if (b < sy) {
+        T *y = Y + b * ldy + i;
+        *y = T((beta == 0.0 ? 0.0 : beta * double(*y)) + acc[b]);
+      }
+  }
+}
+
 int check_block(int precision, long n, int sx, int sy, long ldx, long ldy) {
   if (precision != SPME_FP32 && precision != SPME_FP64 && precision != SPME_FP64_PLAIN)
     return fail(-1, "precision must be SPME_FP32, SPME_FP64 or SPME_FP64_PLAIN");
   if (n < 1) return fail(-1, "n must be positive");
-  if (sx < 1 || sx > BS || sy < 1 || sy > BS)
-    return fail(-1, "block widths must be in [1, %d]", BS);
+  if (sx < 1 || sx > BX || sy < 1 || sy > BS)
+    return fail(-1, "block widths must be sx in [1, %d], sy in [1, %d]", BX, BS);
   if (ldx < n || ldy < n) return fail(-1, "leading dimensions must be >= n");
   return 0;
 }
@@ -116,7 +263,8 @@ using namespace spme;
 extern "C" {
 
 long spme_block_gram_worksize(long n, int sx, int sy) {
-  return (long)gram_blocks(n) * sx * sy;
+  if (n < 1 || sx < 1 || sx > BX || sy < 1 || sy > BS) return 0;
+  return (long)gram_parts(n, sx) * sx * sy;
 }
 
 int spme_block_gram(int precision, long n, int sx, int sy, const void *X, long ldx,
@@ -128,13 +276,23 @@ int spme_block_gram(int precision, long n, int sx, int sy, const void *X, long l
   if (work_len < need) return fail(-1, "workspace too small (%ld < %ld doubles)", work_len, need);
   (void)hipGetLastError();
   hipStream_t st = (hipStream_t)stream;
-  const int nb = gram_blocks(n);
-  if (precision == SPME_FP32)
-    hipLaunchKernelGGL(k_block_gram_part<float>, dim3(nb), dim3(256), 0, st, n, sx, sy,
-                       (const float *)X, ldx, (const float *)Y, ldy, work);
-  else
-    hipLaunchKernelGGL(k_block_gram_part<double>, dim3(nb), dim3(256), 0, st, n, sx, sy,
-                       (const double *)X, ldx, (const double *)Y, ldy, work);
+  const int nb = gram_parts(n, sx);
+  if (sx <= BS) {
+    if (precision == SPME_FP32)
+      hipLaunchKernelGGL(k_block_gram_part<float>, dim3(nb), dim3(256), 0, st, n, sx, sy,
+                         (const float *)X, ldx, (const float *)Y, ldy, work);
+    else
+      hipLaunchKernelGGL(k_block_gram_part<double>, dim3(nb), dim3(256), 0, st, n, sx, sy,
+                         (const double *)X, ldx, (const double *)Y, ldy, work);
+  } else {
+    const dim3 grid(nb / 4, (sx + TA - 1) / TA);
+    if (precision == SPME_FP32)
+      hipLaunchKernelGGL(k_block_gram_tall<float>, grid, dim3(256), 0, st, n, sx, sy,
+                         (const float *)X, ldx, (const float *)Y, ldy, work);
+    else
+      hipLaunchKernelGGL(k_block_gram_tall<double>, grid, dim3(256), 0, st, n, sx, sy,
+                         (const double *)X, ldx, (const double *)Y, ldy, work);
+  }
   hipLaunchKernelGGL(k_block_gram_reduce, dim3(sx * sy), dim3(64), 0, st, nb, sx * sy, work, C);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : fail(-2, "block gram launch: %s", hipGetErrorString(e));
@@ -144,16 +302,31 @@ int spme_block_update(int precision, long n, int sx, int sy, const void *X, long
                       const double *A, double beta, void *Y, long ldy, void *stream) {
   if (int rc = check_block(precision, n, sx, sy, ldx, ldy)) return rc;
   if (!X || !A || !Y) return fail(-1, "null pointer");
-  if (X == Y) return fail(-1, "X and Y must not alias");
+  {  // the spans [first, last element] of X and Y must not overlap
+    const size_t tsz = precision == SPME_FP32 ? sizeof(float) : sizeof(double);
+    const char *x0 = (const char *)X, *y0 = (const char *)Y;
+    const char *x1 = x0 + ((size_t)(sx - 1) * ldx + n) * tsz;
+    const char *y1 = y0 + ((size_t)(sy - 1) * ldy + n) * tsz;
+    if (x0 < y1 && y0 < x1) return fail(-1, "X and Y must not alias (overlapping spans)");
+  }
   (void)hipGetLastError();
   hipStream_t st = (hipStream_t)stream;
   const int nb = int((n + 255) / 256);
-  if (precision == SPME_FP32)
-    hipLaunchKernelGGL(k_block_update<float>, dim3(nb), dim3(256), 0, st, n, sx, sy,
-                       (const float *)X, ldx, A, beta, (float *)Y, ldy);
-  else
-    hipLaunchKernelGGL(k_block_update<double>, dim3(nb), dim3(256), 0, st, n, sx, sy,
-                       (const double *)X, ldx, A, beta, (double *)Y, ldy);
+  if (sx <= BS) {
+    if (precision == SPME_FP32)
+      hipLaunchKernelGGL(k_block_update<float>, dim3(nb), dim3(256), 0, st, n, sx, sy,
+                         (const float *)X, ldx, A, beta, (float *)Y, ldy);
+    else
+      hipLaunchKernelGGL(k_block_update<double>, dim3(nb), dim3(256), 0, st, n, sx, sy,
+                         (const double *)X, ldx, A, beta, (double *)Y, ldy);
+  } else {
+    if (precision == SPME_FP32)
+      hipLaunchKernelGGL(k_block_update_tall<float>, dim3(nb), dim3(256), 0, st, n, sx, sy,
+                         (const float *)X, ldx, A, beta, (float *)Y, ldy);
+    else
+      hipLaunchKernelGGL(k_block_update_tall<double>, dim3(nb), dim3(256), 0, st, n, sx, sy,
+                         (const double *)X, ldx, A, beta, (double *)Y, ldy);
+  }
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : fail(-2, "block update launch: %s", hipGetErrorString(e));
 }

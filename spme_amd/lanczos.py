@@ -8,7 +8,7 @@ wave space) and a block Z of s vectors (rows of a [s][n] array):
 
     Z = Q_1 R_0                                 (Cholesky QR, twice)
     W = M Q_j - Q_{j-1} B_{j-1}^T,  A_j = Q_j^T W,  W -= Q_j A_j
-    W -= Q_i (Q_i^T W) for every i <= j          (full reorthogonalisation)
+    W -= Q (Q^T W),  Q = [Q_1 .. Q_j]            (full reorthogonalisation)
     W = Q_{j+1} B_j                             (Cholesky QR, twice)
     T_m = block tridiagonal (A_j on the diagonal, B_j below, B_j^T above)
     M^{1/2} Z ~ [Q_1 .. Q_m] (T_m^{1/2})[:, :s] R_0
@@ -16,8 +16,12 @@ wave space) and a block Z of s vectors (rows of a [s][n] array):
 T_m^{1/2} comes from the eigendecomposition of the small (s m) x (s m)
 matrix on the host; the iteration stops when the coefficient block changes
 by less than tol (relative Frobenius norm) from one step to the next.  The
-tall-skinny products run as the library's HIP kernels (spme_block_gram,
-spme_block_update), the operator as the engine's batched apply.
+blocks Q_i are consecutive rows of one basis array, so the
+reorthogonalisation is one Gram product and one update over all earlier
+rows (classical Gram-Schmidt as a second pass after the recurrence's own
+projections), and the result one update.  The tall-skinny products run as
+the library's HIP kernels (spme_block_gram, spme_block_update: up to 4096
+basis rows per call), the operator as the engine's batched apply.
 """
 from __future__ import annotations
 
@@ -27,6 +31,9 @@ from dataclasses import dataclass
 import numpy as np
 
 from .lib import SPME_FP32, SPME_FULL, SpmeError, check, load
+
+
+TALL = 4096   # most rows of X in one spme_block_gram / spme_block_update
 
 
 def _ptr(t) -> ctypes.c_void_p:
@@ -51,11 +58,20 @@ class HipBlockOps:
     def _stream(self):
         return ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
 
+    def _workspace(self, sx: int, sy: int):
+        ws = int(load().spme_block_gram_worksize(self.n, sx, sy))
+        if ws > self._work.numel():
+            self._work = self.torch.empty(ws, dtype=self.torch.float64, device=self.device)
+        return self._work
+
     def gram_dev(self, X, Y, out=None):
-        """X Y^T as an FP64 device tensor [sx][sy], without waiting for it."""
+        """X Y^T as an FP64 device tensor [sx][sy], without waiting for it
+        (sx <= 4096 rows of X, sy <= 16 of Y)."""
         sx, sy = X.shape[0], Y.shape[0]
-        C = out if out is not None else self.torch.empty(sx * sy, dtype=self.torch.float64,
-                                                         device=self.device)
+        if out is None or out.numel() < sx * sy:
+            out = self.torch.empty(sx * sy, dtype=self.torch.float64, device=self.device)
+        C = out
+        self._workspace(sx, sy)
         check(load().spme_block_gram(self.precision, self.n, sx, sy, _ptr(X), X.stride(0),
                                      _ptr(Y), Y.stride(0), _ptr(C), _ptr(self._work),
                                      self._work.numel(), self._stream()), "spme_block_gram")
@@ -139,9 +155,10 @@ class BlockLanczos:
         self.apply, self.ops, self.s = apply, ops, int(s)
         self.tol, self.max_iter, self.reorth = float(tol), int(max_iter), bool(reorth)
 
-    def _orth(self, W, scale: float):
+    def _orth(self, W, scale: float, dst=None):
         """V ([k][n], orthonormal rows) and R (k x w) with W = V^T R in the
-        rows' sense (w_b = sum_a v_a R[a][b]); k = numerical rank of W."""
+        rows' sense (w_b = sum_a v_a R[a][b]); k = numerical rank of W.  V is
+        written to the first k rows of dst when given."""
         ops = self.ops
         G = ops.gram(W, W)
         sig2, U = np.linalg.eigh(0.5 * (G + G.T))
@@ -160,7 +177,7 @@ class BlockLanczos:
             R2 = np.linalg.cholesky(0.5 * (G2 + G2.T)).T
         except np.linalg.LinAlgError as exc:
             raise SpmeError("block Lanczos: orthogonalisation failed") from exc
-        V = ops.empty(k)
+        V = dst[:k] if dst is not None else ops.empty(k)
         ops.update(V1, np.linalg.inv(R2), 0.0, V)
         return V, R2 @ R
 
@@ -183,12 +200,16 @@ class BlockLanczos:
         s, ops = self.s, self.ops
         if Z.shape[0] != s:
             raise SpmeError(f"Z must have {s} rows")
-        Q, R0 = self._orth(Z, 0.0)
+        # Q_1, Q_2, ... as consecutive rows of one array (at most s rows per
+        # step and max_iter steps)
+        basis = ops.empty(s * self.max_iter)
+        Q, R0 = self._orth(Z, 0.0, basis)
         U = out if out is not None else ops.empty(s)
         if Q is None:  # Z = 0
             ops.update(Z, np.zeros((s, s)), 0.0, U)
             return U, LanczosInfo(0, 0.0, True, 0.0, 0.0, 0, True)
         Qs, As, Bs = [Q], [], []
+        rows = Q.shape[0]     # basis rows in use
         scale, coef_prev, change = 0.0, None, np.inf
         invariant = False
         next_check = 1
@@ -205,11 +226,12 @@ class BlockLanczos:
             As.append(A)
             scale = max(scale, float(np.abs(np.linalg.eigvalsh(A)).max()))
             if self.reorth:
-                # W -= Q_i (Q_i^T W), block by block; with device-side Gram
-                # products nothing waits on the host inside this loop
+                # W -= Q (Q^T W) over all basis rows, TALL rows per call; with
+                # device-side Gram products nothing here waits on the host
                 dev = getattr(ops, "gram_dev", None)
-                for Qi in Qs:
-                    ops.update(Qi, -(dev(Qi, W) if dev else ops.gram(Qi, W)), 1.0, W)
+                for a in range(0, rows, TALL):
+                    Qa = basis[a:min(rows, a + TALL)]
+                    ops.update(Qa, -(dev(Qa, W) if dev else ops.gram(Qa, W)), 1.0, W)
             m = j + 1
             # the (s m)^3 eigendecomposition is checked every step while it is
             # cheap, then every m/4 steps (the change is then measured over
@@ -225,19 +247,22 @@ class BlockLanczos:
                 next_check = m + (1 if m < 8 else max(1, m // 4))
                 if change < self.tol or m == self.max_iter:
                     break
-            Qn, B = self._orth(W, scale)
+            Qn, B = self._orth(W, scale, basis[rows:rows + kj])
             if Qn is None:       # invariant space: the result is exact
                 invariant = True
                 break
             Qs.append(Qn)
             Bs.append(B)
+            rows += Qn.shape[0]
         if checked != len(As):
             coef, lo, hi, off = self._coefficients(As, Bs, R0)
         m = len(As)
         info = LanczosInfo(m, change, invariant or change < self.tol, lo, hi, int(off[-1]),
                            invariant)
-        for i in range(m):
-            ops.update(Qs[i], coef[off[i]:off[i + 1]], 0.0 if i == 0 else 1.0, U)
+        used = int(off[-1])   # rows of Q_1 .. Q_m
+        for a in range(0, used, TALL):
+            b = min(used, a + TALL)
+            ops.update(basis[a:b], coef[a:b], 0.0 if a == 0 else 1.0, U)
         return U, info
 
 

@@ -216,3 +216,46 @@ def test_brownian_displacement_parts():
     q2 = (Z * MZ).sum(dim=1)
     assert float(((q1 - q2).abs() / q2).max()) < 1e-6
     eng.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+def test_tall_block_kernels_match_numpy(prec):
+    """The tall form (X up to 4096 rows: every earlier block of the basis
+    at once) against numpy in FP64, deterministic, with its limits."""
+    torch = _torch()
+    from spme_amd import engine as E
+    from spme_amd.lanczos import HipBlockOps
+    print(f"case: tall block gram/update {prec}", flush=True)
+    P = E.SPME_FP32 if prec == "fp32" else E.SPME_FP64
+    dt = torch.float32 if prec == "fp32" else torch.float64
+    n, ld = 10007, 10040
+    ops = HipBlockOps(P, n)
+    rng = np.random.default_rng(9)
+    for sx, sy in [(17, 16), (100, 7), (1000, 16)]:
+        X = rng.standard_normal((sx, ld))
+        Y = rng.standard_normal((sy, ld))
+        A = rng.standard_normal((sx, sy))
+        if prec == "fp32":
+            X = X.astype(np.float32).astype(np.float64)
+            Y = Y.astype(np.float32).astype(np.float64)
+        dX = torch.tensor(X, device="cuda", dtype=dt)
+        dY = torch.tensor(Y, device="cuda", dtype=dt)
+        G1 = ops.gram_dev(dX[:, :n], dY[:, :n]).cpu().numpy()
+        G2 = ops.gram_dev(dX[:, :n], dY[:, :n]).cpu().numpy()
+        np.testing.assert_array_equal(G1, G2)                     # deterministic
+        Gref = X[:, :n] @ Y[:, :n].T
+        assert np.abs(G1 - Gref).max() < 1e-11 * np.abs(Gref).max(), (sx, sy)
+        tol = 1e-6 if prec == "fp32" else 1e-13
+        ops.update(dX[:, :n], A, 1.5, dY[:, :n])
+        ref = 1.5 * Y[:, :n] + A.T @ X[:, :n]
+        got = dY.double().cpu().numpy()
+        assert np.abs(got[:, :n] - ref).max() < tol * np.abs(ref).max(), (sx, sy)
+        np.testing.assert_array_equal(got[:, n:], Y[:, n:])
+    # an update whose output rows lie inside X's span is refused
+    with pytest.raises(SpmeError, match="alias"):
+        ops.update(dX[:40, :n], A[:40, :4], 0.0, dX[30:34, :n])
+    big = torch.zeros(4097, 8, device="cuda", dtype=dt)
+    small = torch.zeros(2, 8, device="cuda", dtype=dt)
+    with pytest.raises(SpmeError, match="block widths"):
+        HipBlockOps(P, 8).gram_dev(big, small)
