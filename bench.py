@@ -65,7 +65,9 @@ def parse():
                     help="skip the uneven multi-system leg")
     ap.add_argument("--no-brownian", action="store_true",
                     help="skip the block-Lanczos Brownian displacement leg")
-    ap.add_argument("--cpu-rhs", type=int, default=1)
+    ap.add_argument("--cpu-rhs", type=int, default=6,
+                    help="right-hand sides of the CPU baseline sample (about 1.8 s each at "
+                         "p=8 on 16 host threads)")
     ap.add_argument("--pmc-run", action="store_true",
                     help="short run for hardware-counter passes: a plain copy of "
                          "known size (calibration), then 2 batched applies")
