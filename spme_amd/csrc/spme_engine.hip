@@ -8,6 +8,7 @@
 // independent.
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -148,6 +149,7 @@ struct _spme_t {
   int nkeys;
   double *pos;  // device positions (N x 3)
   int *base_tmp, *key_tmp, *counts, *cursor, *boff, *perm, *pbase, *err;
+  int *bigq;  // queue of buckets too large for the per-thread sort (count first)
   double *w_tmp;
   void *pw;  // sorted weights [N][3][P]
   void *fs;  // sorted forces [N][3*max_rhs]
@@ -246,7 +248,7 @@ void free_all(_spme_t *e) {
                   e->cursor, e->boff, e->perm, e->pbase, e->err, e->w_tmp, e->pw, e->fs,
                   e->r_key, e->r_counts, e->r_cursor, e->r_off, e->r_perm, e->r_pos,
                   e->r_fs, e->r_cnt, e->r_width, e->r_size, e->r_soff, e->r_col, e->r_blk,
-                  e->h_in, e->h_out, e->dP, e->dind, e->iperm, e->r_iperm};
+                  e->h_in, e->h_out, e->dP, e->dind, e->iperm, e->r_iperm, e->bigq};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &r : e->pending) {
@@ -301,6 +303,7 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
   E_TRY(hipMalloc(&e->key_tmp, sizeof(int) * N));
   E_TRY(hipMalloc(&e->counts, sizeof(int) * e->nkeys));
   E_TRY(hipMalloc(&e->cursor, sizeof(int) * e->nkeys));
+  E_TRY(hipMalloc(&e->bigq, sizeof(int) * bucket_queue_ints(N)));
   E_TRY(hipMalloc(&e->boff, sizeof(int) * (e->nkeys + 1)));
   E_TRY(hipMalloc(&e->perm, sizeof(int) * N));
   E_TRY(hipMalloc(&e->iperm, sizeof(int) * N));
@@ -363,7 +366,8 @@ int build_P(_spme_t *e, hipStream_t st) {
   } else {
     launch_scan(e->counts, e->boff, e->nkeys, st);
     launch_pscatter(e->key_tmp, e->boff, e->cursor, e->perm, N, st);
-    launch_bucket_sort(e->boff, e->perm, e->nkeys, st);
+    if (launch_bucket_sort(e->key_tmp, N, e->boff, e->perm, e->nkeys, e->bigq, st))
+      return fail(-2, "bucket sort launch failed");
     launch_invert(e->perm, e->iperm, N, st);
     launch_pgather<T>(e->perm, e->base_tmp, e->w_tmp, e->pbase, (T *)e->pw, N, 3 * P, st);
   }
@@ -374,18 +378,28 @@ int build_P(_spme_t *e, hipStream_t st) {
     launch_cell_key(e->pos, N, e->prm.box_size, e->nc, e->r_key, e->r_counts, st);
     launch_scan(e->r_counts, e->r_off, nc3, st);
     launch_pscatter(e->r_key, e->r_off, e->r_cursor, e->r_perm, N, st);
-    launch_bucket_sort(e->r_off, e->r_perm, nc3, st);
+    if (launch_bucket_sort(e->r_key, N, e->r_off, e->r_perm, nc3, e->bigq, st))
+      return fail(-2, "cell sort launch failed");
     launch_invert(e->r_perm, e->r_iperm, N, st);
     launch_cell_gather(e->r_perm, e->pos, e->r_pos, N, st);
     const double L = e->prm.box_size, rc = e->prm.rc;
     const double cut = rc > 2.0 ? rc : 2.0;
     launch_pair_count(e->r_pos, e->r_off, e->nc, L, cut * cut, N, e->r_cnt, st);
     launch_slice_width(e->r_cnt, N, e->r_width, e->r_size, st);
-    launch_scan(e->r_size, e->r_soff, e->nslices, st);
-    int total = 0;
-    HIP_TRY(hipMemcpyAsync(&total, e->r_soff + e->nslices, sizeof(int), hipMemcpyDeviceToHost,
-                           st));
+    // the entry offsets are ints: sum the slice sizes in 64 bits first and
+    // refuse a block list that would not fit
+    std::vector<int> hw(e->nslices);
+    HIP_TRY(hipMemcpyAsync(hw.data(), e->r_width, sizeof(int) * e->nslices,
+                           hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    long total = 0;
+    for (int w : hw) total += 64L * w;
+    if (total > INT_MAX)
+      return fail(-1,
+                  "real-space block list needs %ld entries (at most %d): the particles are "
+                  "too clustered for rc = %g",
+                  total, INT_MAX, rc);
+    launch_scan(e->r_size, e->r_soff, e->nslices, st);
     if (total > e->r_cap) {
       (void)hipFree(e->r_col);
       (void)hipFree(e->r_blk);

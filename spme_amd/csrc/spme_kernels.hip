@@ -96,11 +96,20 @@ __global__ __launch_bounds__(256) void k_pscatter(const int *__restrict__ key,
 
 // Order each bucket by original particle index, so the sorted layout (and
 // with it every summation order downstream) does not depend on atomics.
+// A bucket of at most SORT_MAX particles is insertion-sorted by one thread;
+// a larger one (clustered input) is queued in big[] for k_bucket_big, so
+// that the cost stays bounded however the particles cluster.
 __global__ __launch_bounds__(256) void k_bucket_sort(const int *__restrict__ boff,
-                                                     int *__restrict__ perm, int nkeys) {
+                                                     int *__restrict__ perm, int nkeys,
+                                                     int *__restrict__ big,
+                                                     int *__restrict__ nbig) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nkeys) return;
   const int a = boff[k], e = boff[k + 1];
+  if (e - a > SORT_MAX) {
+    big[atomicAdd(nbig, 1)] = k;
+    return;
+  }
   for (int j = a + 1; j < e; ++j) {
     const int v = perm[j];
     int m = j - 1;
@@ -109,6 +118,38 @@ __global__ __launch_bounds__(256) void k_bucket_sort(const int *__restrict__ bof
       --m;
     }
     perm[m + 1] = v;
+  }
+}
+
+// The queued buckets: block b rebuilds buckets big[b], big[b + gridDim.x],
+// ... by one stable pass over all keys (a block-wide prefix count of the
+// matches in each 256-index chunk), which lists each bucket's particles in
+// increasing original index.  At most N / (SORT_MAX + 1) buckets qualify,
+// so the pass costs at most N^2 / (SORT_MAX + 1) key reads.
+__global__ __launch_bounds__(256) void k_bucket_big(const int *__restrict__ key, int N,
+                                                    const int *__restrict__ boff,
+                                                    const int *__restrict__ big,
+                                                    const int *__restrict__ nbig,
+                                                    int *__restrict__ perm) {
+  __shared__ int wsum[4];
+  const int nb = *nbig;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int q = blockIdx.x; q < nb; q += gridDim.x) {
+    const int k = big[q];
+    int base = boff[k];
+    for (int i0 = 0; i0 < N; i0 += 256) {
+      const int i = i0 + threadIdx.x;
+      const bool hit = i < N && key[i] == k;
+      const unsigned long long m = __ballot(hit);
+      if (lane == 0) wsum[w] = __popcll(m);
+      __syncthreads();
+      int off = 0;
+      for (int v = 0; v < w; ++v) off += wsum[v];
+      const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+      if (hit) perm[base + off + __popcll(m & ((1ull << lane) - 1))] = i;
+      base += tot;
+      __syncthreads();  // wsum is rewritten by the next chunk
+    }
   }
 }
 
@@ -697,9 +738,15 @@ void launch_pscatter(const int *key, const int *boff, int *cursor, int *perm, in
                      perm, N);
 }
 
-void launch_bucket_sort(const int *boff, int *perm, int nkeys, hipStream_t st) {
+int launch_bucket_sort(const int *key, int N, const int *boff, int *perm, int nkeys, int *bigq,
+                       hipStream_t st) {
+  // bigq: the queue count, then room for N / (SORT_MAX + 1) + 1 buckets
+  if (hipMemsetAsync(bigq, 0, sizeof(int), st) != hipSuccess) return -1;
   hipLaunchKernelGGL(k_bucket_sort, dim3(grid1(nkeys, 256)), dim3(256), 0, st, boff, perm,
-                     nkeys);
+                     nkeys, bigq + 1, bigq);
+  hipLaunchKernelGGL(k_bucket_big, dim3(256), dim3(256), 0, st, key, N, boff, bigq + 1, bigq,
+                     perm);
+  return 0;
 }
 
 template <typename T>

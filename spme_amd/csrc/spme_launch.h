@@ -125,7 +125,11 @@ void launch_pbuild(const double *pos, int N, double L, int K, int P, const WTabl
 void launch_scan(const int *in, int *out, int n, hipStream_t st);
 void launch_pscatter(const int *key, const int *boff, int *cursor, int *perm, int N,
                      hipStream_t st);
-void launch_bucket_sort(const int *boff, int *perm, int nkeys, hipStream_t st);
+// buckets above SORT_MAX particles are rebuilt by a stable pass over the keys
+constexpr int SORT_MAX = 256;
+inline long bucket_queue_ints(int N) { return N / (SORT_MAX + 1) + 2; }
+int launch_bucket_sort(const int *key, int N, const int *boff, int *perm, int nkeys, int *bigq,
+                       hipStream_t st);
 template <typename T>
 void launch_pgather(const int *perm, const int *base_tmp, const double *w_tmp, int *pbase,
                     T *pw, int N, int P3, hipStream_t st);

@@ -10,6 +10,11 @@
   holds all 1,999 others, in the middle of the box and around the corner
   (coordinates below 0 and wrapped).
 
+* extreme clustering: 50,000 particles in a unit cube, so that single sort
+  buckets hold tens of thousands (sorted by a stable pass over the keys,
+  not per thread), and a real-space block list that would need more than
+  2^31 entries is refused with an error.
+
 FP64 bar: relative max-norm difference <= 1e-12; FP32 <= 2e-5 (as the
 other parity tests).
 """
@@ -17,6 +22,7 @@ import numpy as np
 import pytest
 
 from spme_amd import workload as W
+from spme_amd.lib import SpmeError
 
 pytestmark = pytest.mark.gpu
 
@@ -111,3 +117,20 @@ def test_clustered_particles(oracle, E, torch, where, p, K):
     pos = c + _cluster(N, 2.5, seed=7)
     _say(f"case: {N} particles clustered at the {where}, L={L:.1f} K={K} p={p}")
     _check(oracle, E, torch, 0.5, K, p, L, 5.0, pos, W.forces(70, N, 3))
+
+
+def test_extreme_clustering(oracle, E, torch):
+    N, L, K, p, xi = 50_000, 40.0, 32, 6, 0.5
+    pos = 20.0 + np.random.default_rng(11).uniform(-0.5, 0.5, (N, 3))
+    f = W.forces(80, N, 1)
+    _say(f"case: {N} particles in a unit cube, L={L} K={K} p={p}, reciprocal part")
+    ref = oracle.compute_spme(xi, K, p, L, pos, f)
+    got = _apply(E, torch, xi, K, p, L, 0.0, pos, f, E.SPME_FP64, E.SPME_RECIP)
+    d = _rel(got, ref)
+    _say(f"  fp64 vs oracle: {d:.2e}")
+    assert d < FP64_TOL
+    _say("case: the same with rc = 5: N(N-1) block entries, refused")
+    eng = E.Engine(xi, K, p, N, L, rc=5.0, precision=E.SPME_FP64)
+    with pytest.raises(SpmeError, match="too clustered"):
+        eng.set_positions(torch.tensor(pos, device="cuda"))
+    eng.close()
