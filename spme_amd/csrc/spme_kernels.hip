@@ -470,7 +470,9 @@ __global__ __launch_bounds__(NT) void k_fft_xy_fwd(T *__restrict__ grid, int RS,
   }
 }
 
-template <typename T, int K, int NT>
+// NYQ: the Nyquist column x = K/2 comes from its own array (nyq), which
+// the launcher passes exactly when NYQ is set.
+template <typename T, int K, int NT, bool NYQ>
 __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS, long comp_stride,
                                                    int nplanes, const cpx<T> *__restrict__ tw,
                                                    const cpx<T> *__restrict__ nyq) {
@@ -483,7 +485,7 @@ __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS,
   // (K/G units)
   constexpr int UPR = (Kh + G - 1) / G, UPQ = (K / 2) / G;
   static_assert((K / 2) % G == 0 && K % G == 0, "whole units");
-  constexpr int NUNIT = K * UPR > K * UPQ + K / G ? K * UPR : K * UPQ + K / G;
+  constexpr int NUNIT = NYQ ? K * UPQ + K / G : K * UPR;
   constexpr int NU = (NUNIT + NT - 1) / NT;
   __shared__ cpx<T> buf[S::NCPX];
   __shared__ cpx<T> s_tw[K];
@@ -492,20 +494,21 @@ __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS,
   int tid = threadIdx.x;
   const int Khp = RS / 2;
   C16 rv[NU];
-  const int upr = nyq ? UPQ : UPR;               // row units
-  const int nunit = nyq ? K * UPQ + K / G : K * UPR;
+  constexpr int upr = NYQ ? UPQ : UPR;  // row units (a constant: divisions by it are cheap)
   auto fetch = [&](int q) {
     const int comp = q / K, z = q - comp * K;
     const cpx<T> *spec =
         reinterpret_cast<const cpx<T> *>(grid + comp * comp_stride) + (size_t)z * K * Khp;
-    const cpx<T> *nq = nyq ? nyq + ((size_t)comp * K + z) * K : nullptr;
+    const cpx<T> *nq = NYQ ? nyq + ((size_t)comp * K + z) * K : nullptr;
 #pragma unroll
     for (int r = 0; r < NU; ++r) {
       const int u = tid + NT * r;
-      if (u < nunit) {
+      if (u < NUNIT) {
         const int y = u / upr, g = u - y * upr;
-        rv[r] = y < K ? *reinterpret_cast<const C16 *>(spec + (size_t)y * Khp + G * g)
-                      : *reinterpret_cast<const C16 *>(nq + G * (u - K * upr));
+        if (!NYQ || y < K)
+          rv[r] = *reinterpret_cast<const C16 *>(spec + (size_t)y * Khp + G * g);
+        else
+          rv[r] = *reinterpret_cast<const C16 *>(nq + G * (u - K * upr));
       }
     }
   };
@@ -513,9 +516,9 @@ __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS,
 #pragma unroll
     for (int r = 0; r < NU; ++r) {
       const int u = tid + NT * r;
-      if (u < nunit) {
+      if (u < NUNIT) {
         const int y = u / upr, g = u - y * upr;
-        if (y < K) {
+        if (!NYQ || y < K) {
 #pragma unroll
           for (int i = 0; i < G; ++i)
             if (G * g + i < Kh) buf[y * Kh + G * g + i] = rv[r].v[i];
@@ -878,8 +881,12 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
     hipLaunchKernelGGL((k_fft_z_infl<T, K, ZS::XW, ZS::NT>), dim3(nzb), dim3(ZS::NT), 0, st,
                        a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw, a.nyq);
     mark(2);
-    hipLaunchKernelGGL((k_fft_xy_inv<T, K, NTXY>), dim3(grid), dim3(NTXY), 0, st, a.grid, a.RS,
-                       a.comp_stride, nplanes, a.tw, a.nyq);
+    if (a.nyq)
+      hipLaunchKernelGGL((k_fft_xy_inv<T, K, NTXY, true>), dim3(grid), dim3(NTXY), 0, st, a.grid,
+                         a.RS, a.comp_stride, nplanes, a.tw, a.nyq);
+    else
+      hipLaunchKernelGGL((k_fft_xy_inv<T, K, NTXY, false>), dim3(grid), dim3(NTXY), 0, st,
+                         a.grid, a.RS, a.comp_stride, nplanes, a.tw, a.nyq);
     mark(4);
   } else {
     hipLaunchKernelGGL((k_fft_x_r2c<T, K, S::NLX, NT>), dim3(grid1(npairs, S::NLX)), dim3(NT),
