@@ -351,8 +351,9 @@ __global__ __launch_bounds__(NT) void k_fft_xy_fwd(T *__restrict__ grid, int RS,
     const T *base = grid + comp * comp_stride + (size_t)z * K * RS;
 #pragma unroll
     for (int r = 0; r < NU; ++r) {
+      const bool full = NT * (r + 1) <= NUNIT;  // a constant once unrolled
       const int u = tid + NT * r;
-      if (u < NUNIT) {
+      if (full || u < NUNIT) {
         const int l = u / UPL, g = u - l * UPL;
         ra[r] = *reinterpret_cast<const V *>(base + (size_t)(2 * l) * RS + G * g);
         rb[r] = *reinterpret_cast<const V *>(base + (size_t)(2 * l + 1) * RS + G * g);
@@ -362,8 +363,9 @@ __global__ __launch_bounds__(NT) void k_fft_xy_fwd(T *__restrict__ grid, int RS,
   auto commit = [&]() {
 #pragma unroll
     for (int r = 0; r < NU; ++r) {
+      const bool full = NT * (r + 1) <= NUNIT;
       const int u = tid + NT * r;
-      if (u < NUNIT) {
+      if (full || u < NUNIT) {
         const int l = u / UPL, g = u - l * UPL;
 #pragma unroll
         for (int i = 0; i < G; ++i) buf[l * LP + G * g + i] = {ra[r].v[i], rb[r].v[i]};
@@ -502,10 +504,13 @@ __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS,
     const cpx<T> *nq = NYQ ? nyq + ((size_t)comp * K + z) * K : nullptr;
 #pragma unroll
     for (int r = 0; r < NU; ++r) {
+      // full(r): every thread's unit r exists; rows(r): all of them are row
+      // units (constants once the loop is unrolled: no branch where true)
+      const bool full = NT * (r + 1) <= NUNIT, rows = NT * (r + 1) <= K * upr;
       const int u = tid + NT * r;
-      if (u < NUNIT) {
+      if (full || u < NUNIT) {
         const int y = u / upr, g = u - y * upr;
-        if (!NYQ || y < K)
+        if (!NYQ || rows || y < K)
           rv[r] = *reinterpret_cast<const C16 *>(spec + (size_t)y * Khp + G * g);
         else
           rv[r] = *reinterpret_cast<const C16 *>(nq + G * (u - K * upr));
@@ -515,13 +520,15 @@ __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS,
   auto commit = [&]() {
 #pragma unroll
     for (int r = 0; r < NU; ++r) {
+      const bool full = NT * (r + 1) <= NUNIT, rows = NT * (r + 1) <= K * upr;
       const int u = tid + NT * r;
-      if (u < NUNIT) {
+      if (full || u < NUNIT) {
         const int y = u / upr, g = u - y * upr;
-        if (!NYQ || y < K) {
+        if (!NYQ || rows || y < K) {
+          // with the Nyquist column apart, row units cover x < K/2 < Kh
 #pragma unroll
           for (int i = 0; i < G; ++i)
-            if (G * g + i < Kh) buf[y * Kh + G * g + i] = rv[r].v[i];
+            if (NYQ || G * g + i < Kh) buf[y * Kh + G * g + i] = rv[r].v[i];
         } else {  // Nyquist units: rows G w .. G w + G - 1 of column K/2
           const int w = u - K * upr;
 #pragma unroll
