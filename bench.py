@@ -45,7 +45,7 @@ ROCPROF_NAME = {
     "spread": "k_spread<float, 8, 12, 8, 8>",
     "fft_x_fwd": "k_fft_xy_fwd<float, 192, 768>",
     "fft_z_infl": "k_fft_z_infl<float, 192, 16, 512>",
-    "fft_x_inv": "k_fft_xy_inv<float, 192, 768>",
+    "fft_x_inv": "k_fft_xy_inv<float, 192, 768, true>",
     "interp": "k_interp<float, 8, 4, 16, 1024>",
     "permute_real": "k_permute_in<float>",
     "real": "k_real_spmv<float, 48>",
