@@ -43,7 +43,7 @@ MARGIN = 1.5           # the selection is asked for TOL / MARGIN
 ROCPROF_NAME = {
     "permute": "k_permute_in<float>",
     "spread": "k_spread<float, 8, 12, 8, 8>",
-    "fft_x_fwd": "k_fft_xy_fwd<float, 192, 768>",
+    "fft_x_fwd": "k_fft_xy_fwd<float, 192, 768, true>",
     "fft_z_infl": "k_fft_z_infl<float, 192, 16, 512>",
     "fft_x_inv": "k_fft_xy_inv<float, 192, 768, true>",
     "interp": "k_interp<float, 8, 4, 16, 1024>",

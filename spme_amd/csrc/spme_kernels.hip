@@ -330,7 +330,9 @@ __device__ __forceinline__ void xy_pack(cpx<T> *line, int k) {
   if (k > 0 && k < K / 2) line[K - k] = {A.x + B.y, B.x - A.y};
 }
 
-template <typename T, int K, int NT>
+// NYQ: the Nyquist column x = K/2 goes to its own array (nyq), which the
+// launcher passes exactly when NYQ is set.
+template <typename T, int K, int NT, bool NYQ>
 __global__ __launch_bounds__(NT) void k_fft_xy_fwd(T *__restrict__ grid, int RS, long comp_stride,
                                                    int nplanes, const cpx<T> *__restrict__ tw,
                                                    cpx<T> *__restrict__ nyq) {
@@ -447,25 +449,31 @@ __global__ __launch_bounds__(NT) void k_fft_xy_fwd(T *__restrict__ grid, int RS,
       for (int k1 = 0; k1 < N1; ++k1) buf[(k1 * N2 + n2) * Kh + x] = a[k1];
     }
     __syncthreads();
-    // y, step B straight to the spectrum rows
+    // y, step B straight to the spectrum rows: the columns that stay in
+    // the rows (all Kh, or the K/2 below the Nyquist column), then, with a
+    // separate Nyquist array, that column's N1 tasks in the first lanes
     {
       const int comp = q / K, z = q - comp * K;
       cpx<T> *spec = reinterpret_cast<cpx<T> *>(grid + comp * comp_stride) + (size_t)z * K * Khp;
-      // the Nyquist column goes to its own array when there is one
-      cpx<T> *nq = nyq ? nyq + ((size_t)comp * K + z) * K : nullptr;
-      for (int t = tid; t < Kh * N1; t += NT) {
-        const int x = t % Kh, k1 = t / Kh;
+      constexpr int XR = NYQ ? K / 2 : Kh;
+      for (int t = tid; t < XR * N1; t += NT) {
+        const int x = t % XR, k1 = t / XR;
         cpx<T> bb[N2];
 #pragma unroll
         for (int n2 = 0; n2 < N2; ++n2) bb[n2] = buf[(k1 * N2 + n2) * Kh + x];
         reg_fft<N2, -1>(bb);
-        if (nq && x == K / 2) {
 #pragma unroll
-          for (int k2 = 0; k2 < N2; ++k2) nq[k1 + N1 * k2] = bb[k2];
-        } else {
+        for (int k2 = 0; k2 < N2; ++k2) spec[(size_t)(k1 + N1 * k2) * Khp + x] = bb[k2];
+      }
+      if (NYQ && tid < N1) {
+        cpx<T> *nq = nyq + ((size_t)comp * K + z) * K;
+        const int k1 = tid;
+        cpx<T> bb[N2];
 #pragma unroll
-          for (int k2 = 0; k2 < N2; ++k2) spec[(size_t)(k1 + N1 * k2) * Khp + x] = bb[k2];
-        }
+        for (int n2 = 0; n2 < N2; ++n2) bb[n2] = buf[(k1 * N2 + n2) * Kh + K / 2];
+        reg_fft<N2, -1>(bb);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; ++k2) nq[k1 + N1 * k2] = bb[k2];
       }
     }
     __syncthreads();
@@ -882,8 +890,12 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
     const int nplanes = a.ncomp * K;
     const int per_cu = std::max(1, std::min(2, int((160 * 1024) / XYShape<T, K>::LDS)));
     const int grid = std::min(nplanes, std::max(1, a.ncu) * per_cu);
-    hipLaunchKernelGGL((k_fft_xy_fwd<T, K, NTXY>), dim3(grid), dim3(NTXY), 0, st, a.grid, a.RS,
-                       a.comp_stride, nplanes, a.tw, a.nyq);
+    if (a.nyq)
+      hipLaunchKernelGGL((k_fft_xy_fwd<T, K, NTXY, true>), dim3(grid), dim3(NTXY), 0, st, a.grid,
+                         a.RS, a.comp_stride, nplanes, a.tw, a.nyq);
+    else
+      hipLaunchKernelGGL((k_fft_xy_fwd<T, K, NTXY, false>), dim3(grid), dim3(NTXY), 0, st,
+                         a.grid, a.RS, a.comp_stride, nplanes, a.tw, a.nyq);
     mark(0);
     hipLaunchKernelGGL((k_fft_z_infl<T, K, ZS::XW, ZS::NT>), dim3(nzb), dim3(ZS::NT), 0, st,
                        a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw, a.nyq);
