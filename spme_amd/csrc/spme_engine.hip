@@ -366,7 +366,7 @@ int build_P(_spme_t *e, hipStream_t st) {
   } else {
     launch_scan(e->counts, e->boff, e->nkeys, st);
     launch_pscatter(e->key_tmp, e->boff, e->cursor, e->perm, N, st);
-    if (launch_bucket_sort(e->key_tmp, N, e->boff, e->perm, e->nkeys, e->bigq, st))
+    if (launch_bucket_sort(e->key_tmp, N, e->boff, e->perm, e->nkeys, e->bigq, e->iperm, st))
       return fail(-2, "bucket sort launch failed");
     launch_invert(e->perm, e->iperm, N, st);
     launch_pgather<T>(e->perm, e->base_tmp, e->w_tmp, e->pbase, (T *)e->pw, N, 3 * P, st);
@@ -378,7 +378,7 @@ int build_P(_spme_t *e, hipStream_t st) {
     launch_cell_key(e->pos, N, e->prm.box_size, e->nc, e->r_key, e->r_counts, st);
     launch_scan(e->r_counts, e->r_off, nc3, st);
     launch_pscatter(e->r_key, e->r_off, e->r_cursor, e->r_perm, N, st);
-    if (launch_bucket_sort(e->r_key, N, e->r_off, e->r_perm, nc3, e->bigq, st))
+    if (launch_bucket_sort(e->r_key, N, e->r_off, e->r_perm, nc3, e->bigq, e->r_iperm, st))
       return fail(-2, "cell sort launch failed");
     launch_invert(e->r_perm, e->r_iperm, N, st);
     launch_cell_gather(e->r_perm, e->pos, e->r_pos, N, st);

@@ -96,29 +96,29 @@ __global__ __launch_bounds__(256) void k_pscatter(const int *__restrict__ key,
 
 // Order each bucket by original particle index, so the sorted layout (and
 // with it every summation order downstream) does not depend on atomics.
-// A bucket of at most SORT_MAX particles is insertion-sorted by one thread;
-// a larger one (clustered input) is queued in big[] for k_bucket_big, so
-// that the cost stays bounded however the particles cluster.
-__global__ __launch_bounds__(256) void k_bucket_sort(const int *__restrict__ boff,
-                                                     int *__restrict__ perm, int nkeys,
-                                                     int *__restrict__ big,
-                                                     int *__restrict__ nbig) {
+// Particle i of a bucket of at most SORT_MAX particles is placed at its
+// rank, the number of the bucket's particles with a smaller index (one
+// thread per particle, independent loads); a larger bucket (clustered
+// input) is queued in big[] for k_bucket_big, so that the cost stays
+// bounded however the particles cluster.  Both write out[], a second array.
+__global__ __launch_bounds__(256) void k_bucket_rank(const int *__restrict__ key, int N,
+                                                     const int *__restrict__ boff,
+                                                     const int *__restrict__ perm,
+                                                     int *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int k = key[i], a = boff[k], e = boff[k + 1];
+  if (e - a > SORT_MAX) return;
+  int r = 0;
+  for (int m = a; m < e; ++m) r += perm[m] < i;
+  out[a + r] = i;
+}
+
+__global__ __launch_bounds__(256) void k_bucket_queue(const int *__restrict__ boff, int nkeys,
+                                                      int *__restrict__ big,
+                                                      int *__restrict__ nbig) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nkeys) return;
-  const int a = boff[k], e = boff[k + 1];
-  if (e - a > SORT_MAX) {
-    big[atomicAdd(nbig, 1)] = k;
-    return;
-  }
-  for (int j = a + 1; j < e; ++j) {
-    const int v = perm[j];
-    int m = j - 1;
-    while (m >= a && perm[m] > v) {
-      perm[m + 1] = perm[m];
-      --m;
-    }
-    perm[m + 1] = v;
-  }
+  if (k < nkeys && boff[k + 1] - boff[k] > SORT_MAX) big[atomicAdd(nbig, 1)] = k;
 }
 
 // The queued buckets: block b rebuilds buckets big[b], big[b + gridDim.x],
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void k_bucket_big(const int *__restrict__ key,
                                                     const int *__restrict__ boff,
                                                     const int *__restrict__ big,
                                                     const int *__restrict__ nbig,
-                                                    int *__restrict__ perm) {
+                                                    int *__restrict__ out) {
   __shared__ int wsum[4];
   const int nb = *nbig;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -146,7 +146,7 @@ __global__ __launch_bounds__(256) void k_bucket_big(const int *__restrict__ key,
       int off = 0;
       for (int v = 0; v < w; ++v) off += wsum[v];
       const int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-      if (hit) perm[base + off + __popcll(m & ((1ull << lane) - 1))] = i;
+      if (hit) out[base + off + __popcll(m & ((1ull << lane) - 1))] = i;
       base += tot;
       __syncthreads();  // wsum is rewritten by the next chunk
     }
@@ -757,13 +757,18 @@ void launch_pscatter(const int *key, const int *boff, int *cursor, int *perm, in
 }
 
 int launch_bucket_sort(const int *key, int N, const int *boff, int *perm, int nkeys, int *bigq,
-                       hipStream_t st) {
-  // bigq: the queue count, then room for N / (SORT_MAX + 1) + 1 buckets
+                       int *tmp, hipStream_t st) {
+  // bigq: the queue count, then room for N / (SORT_MAX + 1) + 1 buckets;
+  // tmp: N ints, the ordered buckets before they are copied back to perm
   if (hipMemsetAsync(bigq, 0, sizeof(int), st) != hipSuccess) return -1;
-  hipLaunchKernelGGL(k_bucket_sort, dim3(grid1(nkeys, 256)), dim3(256), 0, st, boff, perm,
-                     nkeys, bigq + 1, bigq);
+  hipLaunchKernelGGL(k_bucket_queue, dim3(grid1(nkeys, 256)), dim3(256), 0, st, boff, nkeys,
+                     bigq + 1, bigq);
+  hipLaunchKernelGGL(k_bucket_rank, dim3(grid1(N, 256)), dim3(256), 0, st, key, N, boff, perm,
+                     tmp);
   hipLaunchKernelGGL(k_bucket_big, dim3(256), dim3(256), 0, st, key, N, boff, bigq + 1, bigq,
-                     perm);
+                     tmp);
+  if (hipMemcpyAsync(perm, tmp, sizeof(int) * N, hipMemcpyDeviceToDevice, st) != hipSuccess)
+    return -1;
   return 0;
 }
 

@@ -129,7 +129,7 @@ void launch_pscatter(const int *key, const int *boff, int *cursor, int *perm, in
 constexpr int SORT_MAX = 256;
 inline long bucket_queue_ints(int N) { return N / (SORT_MAX + 1) + 2; }
 int launch_bucket_sort(const int *key, int N, const int *boff, int *perm, int nkeys, int *bigq,
-                       hipStream_t st);
+                       int *tmp, hipStream_t st);
 template <typename T>
 void launch_pgather(const int *perm, const int *base_tmp, const double *w_tmp, int *pbase,
                     T *pw, int N, int P3, hipStream_t st);
