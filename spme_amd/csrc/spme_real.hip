@@ -151,37 +151,57 @@ __global__ __launch_bounds__(64) void k_slice_width(const int *__restrict__ cnt,
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void k_pair_fill(const double *__restrict__ spos,
+// Columns of each row's entries in neighbour order, padded with the row's
+// own index.  The coefficients follow in a second pass over the stored
+// entries only: computed here, the FP64 erfc/exp of a hit would hold up the
+// whole wave on every candidate that one lane accepts (about one in seven).
+__global__ __launch_bounds__(256) void k_pair_cols(const double *__restrict__ spos,
                                                    const int *__restrict__ coff, int nc,
-                                                   double L, double xi, double rc2,
-                                                   double cut2, int N,
+                                                   double L, double cut2, int N,
                                                    const int *__restrict__ width,
                                                    const int *__restrict__ soff,
-                                                   int *__restrict__ col, T *__restrict__ blk,
-                                                   long E) {
+                                                   int *__restrict__ col) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   const int q = s >> 6, l = s & 63;
   const long base = soff[q] + l;
   int k = 0;
-  for_neighbours(spos, coff, nc, L, s, cut2, [&](int j, double ex, double ey, double ez,
-                                                 double r2) {
-    double F, G;
-    pair_coeffs(r2, xi, rc2, F, G);
-    const long e = base + 64L * k;
-    col[e] = j;
-    blk[e] = T(F);
-    blk[E + e] = T(G / r2);
+  for_neighbours(spos, coff, nc, L, s, cut2, [&](int j, double, double, double, double) {
+    col[base + 64L * k] = j;
     ++k;
   });
-  // pad the row with zero blocks on the row's own particle (no branch in the
-  // product loop)
-  for (; k < width[q]; ++k) {
+  for (; k < width[q]; ++k) col[base + 64L * k] = s;
+}
+
+// F and G/r^2 of every stored entry (zero for the padding on the row itself).
+template <typename T>
+__global__ __launch_bounds__(256) void k_pair_coeffs(const double *__restrict__ spos, int N,
+                                                     double L, double xi, double rc2,
+                                                     const int *__restrict__ width,
+                                                     const int *__restrict__ soff,
+                                                     const int *__restrict__ col,
+                                                     T *__restrict__ blk, long E) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const int q = s >> 6, l = s & 63;
+  const long base = soff[q] + l;
+  const double px = spos[3L * s], py = spos[3L * s + 1], pz = spos[3L * s + 2];
+  const double iL = 1.0 / L;
+  const int w = width[q];
+  for (int k = 0; k < w; ++k) {
     const long e = base + 64L * k;
-    col[e] = s;
-    blk[e] = T(0);
-    blk[E + e] = T(0);
+    const int j = col[e];
+    double F = 0.0, G = 0.0, r2 = 1.0;
+    if (j != s) {
+      double ex = px - spos[3L * j], ey = py - spos[3L * j + 1], ez = pz - spos[3L * j + 2];
+      ex -= L * rint(ex * iL);
+      ey -= L * rint(ey * iL);
+      ez -= L * rint(ez * iL);
+      r2 = ex * ex + ey * ey + ez * ez;
+      pair_coeffs(r2, xi, rc2, F, G);
+    }
+    blk[e] = T(F);
+    blk[E + e] = T(G / r2);
   }
 }
 
@@ -231,8 +251,10 @@ template <typename T>
 void launch_pair_fill(const double *spos, const int *coff, int nc, double L, double xi,
                       double rc2, double cut2, int N, const int *width, const int *soff,
                       int *col, T *blk, long E, hipStream_t st) {
-  hipLaunchKernelGGL(k_pair_fill<T>, dim3(grid1(N, 256)), dim3(256), 0, st, spos, coff, nc, L,
-                     xi, rc2, cut2, N, width, soff, col, blk, E);
+  hipLaunchKernelGGL(k_pair_cols, dim3(grid1(N, 256)), dim3(256), 0, st, spos, coff, nc, L,
+                     cut2, N, width, soff, col);
+  hipLaunchKernelGGL(k_pair_coeffs<T>, dim3(grid1(N, 256)), dim3(256), 0, st, spos, N, L, xi,
+                     rc2, width, soff, col, blk, E);
 }
 
 template <typename T, int C>
