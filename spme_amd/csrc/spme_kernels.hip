@@ -159,11 +159,14 @@ __global__ __launch_bounds__(256) void k_pgather(const int *__restrict__ perm,
                                                  const double *__restrict__ w_tmp,
                                                  int *__restrict__ pbase, T *__restrict__ pw,
                                                  int N, int P3) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= N) return;
+  // thread t: weight j = t % P3 of sorted particle s = t / P3, so that a
+  // particle's row of weights is read and written by consecutive lanes
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)N * P3) return;
+  const int s = int(t / P3), j = int(t - (long)s * P3);
   const int i = perm[s];
-  pbase[s] = base_tmp[i];
-  for (int j = 0; j < P3; ++j) pw[(size_t)s * P3 + j] = T(w_tmp[(size_t)i * P3 + j]);
+  if (j == 0) pbase[s] = base_tmp[i];
+  pw[t] = T(w_tmp[(size_t)i * P3 + j]);
 }
 
 // Forces into sorted particle-major order, one or two orders in one pass:
@@ -775,8 +778,8 @@ int launch_bucket_sort(const int *key, int N, const int *boff, int *perm, int nk
 template <typename T>
 void launch_pgather(const int *perm, const int *base_tmp, const double *w_tmp, int *pbase,
                     T *pw, int N, int P3, hipStream_t st) {
-  hipLaunchKernelGGL(k_pgather<T>, dim3(grid1(N, 256)), dim3(256), 0, st, perm, base_tmp, w_tmp,
-                     pbase, pw, N, P3);
+  hipLaunchKernelGGL(k_pgather<T>, dim3(grid1((long)N * P3, 256)), dim3(256), 0, st, perm,
+                     base_tmp, w_tmp, pbase, pw, N, P3);
 }
 
 template <typename T>
