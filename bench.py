@@ -1,5 +1,9 @@
 """Benchmark: RPY mobility products u = M.f per second at N = 200,000,
-K = 192^3, p = 8, 16 right-hand sides in FP32 (block-Lanczos width), beside
+K = 192^3, p = 8, 16 right-hand sides (block-Lanczos width) in the FP32
+precision form: spreading, FFTs, interpolation and the real-space pair sums
+in FP32, tables and setup in FP64, and the final real + reciprocal sum in
+FP64 (SPME_FP32_SUM64: float forces in, double velocities out).  The same
+engine with FP32 outputs is timed beside it.  Beside
 two FP64 paths on the same positions and parameters, one right-hand side
 per apply as the reference's compute_spme loops them
 (old-version/spme.c:631-702):
@@ -39,16 +43,17 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 TOL = 1e-4             # target relative error of the batched form
 MARGIN = 1.5           # the selection is asked for TOL / MARGIN
-# rocprof kernel name of each timed stage (FP32, K = 192, p = 8, 16 RHS)
+# rocprof kernel name of each timed stage (FP32 mesh path with the FP64
+# final sum, K = 192, p = 8, 16 RHS)
 ROCPROF_NAME = {
     "permute": "k_permute_in<float>",
     "spread": "k_spread<float, 8, 12, 8, 8>",
     "fft_x_fwd": "k_fft_xy_fwd<float, 192, 768, true>",
     "fft_z_infl": "k_fft_z_infl<float, 192, 16, 512>",
     "fft_x_inv": "k_fft_xy_inv<float, 192, 768, true>",
-    "interp": "k_interp<float, 8, 4, 16, 1024>",
+    "interp": "k_interp<float, 8, 4, 16, 1024, double>",
     "permute_real": "k_permute_in<float>",
-    "real": "k_real_spmv<float, 48>",
+    "real": "k_real_spmv<float, 48, double>",
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_c3.md")
 
@@ -109,9 +114,9 @@ def pmc_run():
                      dtype=torch.float32)
     xi, rc, _ = c3_params()
     eng = E.Engine(xi, cfg["K"], cfg["p"], cfg["n"], s.L, rc=rc,
-                   precision=E.SPME_FP32, max_rhs=cfg["nrhs"])
+                   precision=E.SPME_FP32_SUM64, max_rhs=cfg["nrhs"])
     eng.set_positions(torch.tensor(s.pos, device="cuda:0"))
-    u = torch.zeros_like(f)
+    u = torch.zeros(f.shape, dtype=torch.float64, device="cuda:0")
     for _ in range(2):
         eng.apply(f, u, parts=E.SPME_FULL)
     torch.cuda.synchronize()
@@ -119,16 +124,20 @@ def pmc_run():
     print("[pmc] done", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(N, K, p, R, s, real_entries):
+def algorithmic_bytes(N, K, p, R, s, real_entries, so=None):
     """Compulsory HBM bytes per launch of each kernel: b_P = 3p*s + 12 per
-    particle, Kh = (K/2+1)K^2, C = 3R components; the real-space SpMV reads
-    each stored pair entry once (2 coefficients + a column index), each
-    position, force and output vector once."""
+    particle, Kh = (K/2+1)K^2, C = 3R components, s bytes per mesh value and
+    input value, so per output value (8 when the final sum is FP64); the
+    real-space SpMV reads each stored pair entry once (2 coefficients + a
+    column index), each position and force once, and reads and writes the
+    output the reciprocal part left."""
+    so = s if so is None else so
     C = 3 * R
     K3 = K ** 3
     Kh = (K // 2 + 1) * K * K
     bP = 3 * p * s + 12
     vec = 3 * N * R * s
+    vout = 3 * N * R * so
     return {
         # input read once, written in both sorted orders (the full operator)
         "permute": 3 * vec + 8 * N,
@@ -138,11 +147,11 @@ def algorithmic_bytes(N, K, p, R, s, real_entries):
         # the influence factors (O(K^2) values) stay in cache
         "fft_z_infl": 2 * Kh * C * 2 * s,
         "fft_x_inv": Kh * C * 2 * s + K3 * C * s,
-        "interp": K3 * C * s + N * bP + vec,
+        "interp": K3 * C * s + N * bP + vout,
         "permute_real": 2 * vec + 4 * N,
         # per stored entry: column index and the coefficients F, G/r^2; the
         # FP64 positions once per particle
-        "real": real_entries * (2 * s + 4) + 3 * vec + 24 * N + 4 * N,
+        "real": real_entries * (2 * s + 4) + vec + 2 * vout + 24 * N + 4 * N,
         # the whole forward + inverse transform as one unit
         "fft_total": 2 * (K3 * C * s + Kh * C * 2 * s),
     }
@@ -297,11 +306,12 @@ def main():
     f64 = W.forces(cfg["seed"], N, R).astype(np.float32).astype(np.float64)
     pos_d = torch.tensor(sysm.pos, device=dev)
     f32_d = torch.tensor(f64, device=dev, dtype=torch.float32)
-    u32_d = torch.zeros_like(f32_d)
+    u64_out = torch.zeros(f32_d.shape, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
 
-    # ---- batched FP32 form -------------------------------------------------
-    eng = E.Engine(xi, K, p, N, L, rc=rc, precision=E.SPME_FP32, max_rhs=R)
+    # ---- batched FP32 form, final sum in FP64 (the headline) ---------------
+    eng = E.Engine(xi, K, p, N, L, rc=rc, precision=E.SPME_FP32_SUM64, max_rhs=R)
     t0 = time.perf_counter()
     eng.set_positions(pos_d)
     torch.cuda.synchronize()
@@ -309,17 +319,16 @@ def main():
     tiles = eng.autotune(nrhs=R, reps=3)
     real_entries = eng.info().real_entries
     for _ in range(args.warmup):
-        eng.apply(f32_d, u32_d, parts=E.SPME_FULL)
+        eng.apply(f32_d, u64_out, parts=E.SPME_FULL)
     torch.cuda.synchronize()
     print(f"[bench] warmup done ({args.warmup} steps), tiles {tiles}", file=sys.stderr,
           flush=True)
     eng.set_timing(True)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     w0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(args.steps):
-        eng.apply(f32_d, u32_d, parts=E.SPME_FULL)
+        eng.apply(f32_d, u64_out, parts=E.SPME_FULL)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - w0
@@ -328,11 +337,28 @@ def main():
     eng.set_timing(False)
     ms_step = ms_total / args.steps
     value = R * args.steps / (ms_total / 1e3)
-    print(f"[bench] fp32 batched: {ms_step:.3f} ms/step, {value:.1f} M.f/s "
+    print(f"[bench] fp32 batched, fp64 sum: {ms_step:.3f} ms/step, {value:.1f} M.f/s "
           f"(wall {wall:.3f}s)", file=sys.stderr, flush=True)
-    u_full = u32_d.double().cpu().numpy()
+    u_full = u64_out.cpu().numpy()
+    eng.close()
 
-    # ---- Brownian displacements: block Lanczos over the batched apply -------
+    # ---- the same with FP32 outputs; Brownian displacements on it ----------
+    u32_d = torch.zeros_like(f32_d)
+    eng = E.Engine(xi, K, p, N, L, rc=rc, precision=E.SPME_FP32, max_rhs=R)
+    eng.set_positions(pos_d)
+    eng.set_tiles(tiles)
+    for _ in range(max(1, args.warmup)):
+        eng.apply(f32_d, u32_d, parts=E.SPME_FULL)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        eng.apply(f32_d, u32_d, parts=E.SPME_FULL)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms_f32out = ev0.elapsed_time(ev1) / args.steps
+    u_f32out = u32_d.double().cpu().numpy()
+    print(f"[bench] fp32 batched, fp32 outputs: {ms_f32out:.3f} ms/step", file=sys.stderr,
+          flush=True)
     brownian = None
     if not args.no_brownian:
         brownian = brownian_leg(eng, f32_d, R, stream)
@@ -388,13 +414,14 @@ def main():
         def rel(u, r):
             mine = u[r].reshape(N, 3)[rows].reshape(-1)
             return float(np.linalg.norm(mine - direct[r]) / np.linalg.norm(direct[r]))
-        errors = {"fp32_batched": max(rel(u_full, r) for r in range(direct.shape[0])),
+        errors = {"fp32_batched_fp64_sum": max(rel(u_full, r) for r in range(direct.shape[0])),
+                  "fp32_batched_fp32_out": max(rel(u_f32out, r) for r in range(direct.shape[0])),
                   "fp64_plain": max(rel(u_plain, r) for r in range(direct.shape[0])),
                   "fp64_compact": max(rel(u_compact, r) for r in range(direct.shape[0])),
                   "rows": int(rows.size), "rhs_checked": int(direct.shape[0])}
 
     # ---- roofline of the dominant kernel -------------------------------------
-    ab = algorithmic_bytes(N, K, p, R, 4, real_entries)
+    ab = algorithmic_bytes(N, K, p, R, 4, real_entries, so=8)
     per = {}
     for name, (ms, n) in stages.items():
         if n:
@@ -457,7 +484,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32 (mesh path; final real + recip sum f64)",
         "data": "synthetic (seeded uniform positions, N(0,1) forces)",
         "config": {"workload": "RPY M.f, N=200000 unit spheres, phi=0.2, "
                                "L=161.199, K=192, p=8, 16 RHS batched, full operator",
@@ -477,6 +504,10 @@ def main():
         "hbm_copy_ceiling": copy_ceiling,
         "kernels": per,
         "tiles": tiles,
+        "fp32_outputs": {"ms_per_step": round(ms_f32out, 4),
+                         "value": round(R / (ms_f32out / 1e3), 2), "unit": "M.f/s",
+                         "what": "the same engine and apply with the final sum and outputs "
+                                 "in FP32 (SPME_FP32)"},
         "plain_fp64": {"value": round(plain_value, 2), "unit": "M.f/s",
                        "structures": "reference: dense p^3 P + int32 offsets, atomic "
                                      "spreading, separate influence and scale passes",

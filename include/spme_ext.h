@@ -31,8 +31,14 @@ extern "C" {
  *                    spreading by atomic adds into a zeroed mesh, one
  *                    right-hand side at a time, influence and 1/K^3 scale as
  *                    passes of their own.  The yardstick for the others.
- *                    No tiles, groups or debug stages. */
-enum { SPME_FP64 = 0, SPME_FP32 = 1, SPME_FP64_PLAIN = 2 };
+ *                    No tiles, groups or debug stages.
+ *   SPME_FP32_SUM64  the SPME_FP32 mesh path (spreading, FFTs, interpolation
+ *                    and the real-space pair sums in FP32; tables and setup
+ *                    in FP64) whose final beta*out + alpha*(recip + real)
+ *                    sum is formed in FP64: inputs are float, outputs
+ *                    double (the reference keeps that sum in FP64,
+ *                    old-version/spme_kernel.c:28-30).  Not for groups. */
+enum { SPME_FP64 = 0, SPME_FP32 = 1, SPME_FP64_PLAIN = 2, SPME_FP32_SUM64 = 3 };
 
 /* Parts of M to apply (bit mask). */
 enum { SPME_RECIP = 1, SPME_REAL = 2, SPME_SELF = 4, SPME_FULL = 7 };
@@ -61,7 +67,8 @@ int spme_engine_set_positions(spme_t *spme, const double *pos, int on_device, vo
 
 /* out[r*ldout + 3i + c] <- beta*out + alpha*(M f_r)[3i + c], r < nrhs,
  * for the parts in `parts`.  in/out are device pointers of the engine's
- * precision (float for SPME_FP32, double for SPME_FP64).  Asynchronous on
+ * precision (float for SPME_FP32, double for SPME_FP64 and SPME_FP64_PLAIN;
+ * for SPME_FP32_SUM64 in is float and out double).  Asynchronous on
  * stream, a hipStream_t (NULL: the default stream). */
 int spme_engine_apply(spme_t *spme, int nrhs, double alpha, const void *in, long ldin,
                       double beta, void *out, long ldout, int parts, void *stream);

@@ -398,14 +398,16 @@ template <int P> struct InterpPhase {
   static constexpr int MAXB = 128;
 };
 
-template <typename T, int P, int CC, int TXY, int NT>
+// TO is the output type: the stencil sum is formed in T, the final
+// beta*out + alpha*sum in TO (double for the FP32 form with an FP64 sum).
+template <typename T, int P, int CC, int TXY, int NT, typename TO = T>
 __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
                                              const int *__restrict__ pbase,
                                              const T *__restrict__ pw,
                                              const int *__restrict__ perm,
                                              const T *__restrict__ grid, long comp_stride,
-                                             int RS, int K, int C, int ZC, T alpha, T beta,
-                                             T *__restrict__ out, long ldout, int lb) {
+                                             int RS, int K, int C, int ZC, TO alpha, TO beta,
+                                             TO *__restrict__ out, long ldout, int lb) {
   static_assert(CC * sizeof(T) == 16, "16-byte component groups");
   using V = Vec16<T>;
   constexpr int S = InterpPhase<P>::S, MAXB = InterpPhase<P>::MAXB;
@@ -609,8 +611,8 @@ __device__ __forceinline__ void interp_block(const int *__restrict__ boff,
       const int cl = lane & 15;
       if (kon && cl < ncc) {
         const int comp = cc0 + cl, r = comp / 3, d = comp - 3 * r;
-        T *u = out + r * ldout + 3L * s_perm[k] + d;
-        *u = beta == T(0) ? alpha * tot : beta * *u + alpha * tot;
+        TO *u = out + r * ldout + 3L * s_perm[k] + d;
+        *u = beta == TO(0) ? alpha * TO(tot) : beta * *u + alpha * TO(tot);
       }
     }
   };
@@ -1006,15 +1008,16 @@ __device__ __forceinline__ void load_n(const T *__restrict__ p, T (&v)[N]) {
   }
 }
 
-template <typename T, int C>
+// TO: output type (see interp_block); the pair sums are formed in T.
+template <typename T, int C, typename TO = T>
 __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
                                           const int *__restrict__ width,
                                           const int *__restrict__ soff,
                                           const int *__restrict__ col,
                                           const T *__restrict__ blk, long E,
                                           const double *__restrict__ spos, double L, T self,
-                                          const T *__restrict__ fs, T alpha, T beta, int accum,
-                                          T *__restrict__ out, long ldout, int s, int l) {
+                                          const T *__restrict__ fs, TO alpha, TO beta, int accum,
+                                          TO *__restrict__ out, long ldout, int s, int l) {
   constexpr int RPL = RealShape<C>::RPL, NV = 3 * RPL;
   const int q = s >> 6, ls = s & 63;
   const int r0 = l * RPL;
@@ -1051,11 +1054,11 @@ __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
   const long po = 3L * perm[s];
 #pragma unroll
   for (int r = 0; r < RPL; ++r) {
-    T *u = out + (r0 + r) * ldout + po;
+    TO *u = out + (r0 + r) * ldout + po;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      const T prev = accum ? u[d] : (beta == T(0) ? T(0) : beta * u[d]);
-      u[d] = prev + alpha * a[3 * r + d];
+      const TO prev = accum ? u[d] : (beta == TO(0) ? TO(0) : beta * u[d]);
+      u[d] = prev + alpha * TO(a[3 * r + d]);
     }
   }
 }

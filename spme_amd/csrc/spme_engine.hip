@@ -234,8 +234,9 @@ int validate(const spme_params_t *p) {
     return fail(-1, "box_size must be positive");
   if (!(p->xi > 0.0) || !std::isfinite(p->xi)) return fail(-1, "xi must be positive");
   if (p->precision != SPME_FP64 && p->precision != SPME_FP32 &&
-      p->precision != SPME_FP64_PLAIN)
-    return fail(-1, "precision must be SPME_FP64, SPME_FP32 or SPME_FP64_PLAIN");
+      p->precision != SPME_FP64_PLAIN && p->precision != SPME_FP32_SUM64)
+    return fail(-1, "precision must be SPME_FP64, SPME_FP32, SPME_FP64_PLAIN or "
+                    "SPME_FP32_SUM64");
   if (p->max_rhs < 1 || p->max_rhs > 16) return fail(-1, "max_rhs must be in [1, 16]");
   // the spreading stages particle data by 32-bit dword offsets into the
   // sorted force and weight arrays (at most N * 48 values of 2 dwords)
@@ -275,7 +276,8 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
   e->Khp = (e->Kh + 15) / 16 * 16;
   e->RS = 2 * e->Khp;
   e->comp_stride = (long)e->K * e->K * e->RS;
-  e->tsz = e->prec == SPME_FP32 ? 4 : 8;
+  // the mesh path runs in FP32 for SPME_FP32 and SPME_FP32_SUM64
+  e->tsz = (e->prec == SPME_FP32 || e->prec == SPME_FP32_SUM64) ? 4 : 8;
   const int K = e->K, N = e->N, C = 3 * e->max_rhs;
   auto bail = [&](int rc) {
     free_all(e);
@@ -293,9 +295,9 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
   E_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
   // the plain path transforms one right-hand side (3 components) at a time
   E_TRY(hipMalloc(&e->grid, (size_t)(e->dense ? 3 : C) * e->comp_stride * e->tsz));
-  const bool fused = e->prec == SPME_FP32 ? fft_fused<float>(K) : fft_fused<double>(K);
+  const bool fused = e->tsz == 4 ? fft_fused<float>(K) : fft_fused<double>(K);
   if (!e->dense && fused) E_TRY(hipMalloc(&e->nyq, (size_t)C * K * K * 2 * e->tsz));
-  int rc = e->prec == SPME_FP32 ? upload_tables<float>(e) : upload_tables<double>(e);
+  int rc = e->tsz == 4 ? upload_tables<float>(e) : upload_tables<double>(e);
   if (rc) return bail(rc);
   e->nkeys = (K / 8) * (K / 8) * K;
   E_TRY(hipMalloc(&e->pos, sizeof(double) * 3 * N));
@@ -414,7 +416,7 @@ int build_P(_spme_t *e, hipStream_t st) {
     e->r_E = e->r_cap;
     e->r_used = total;
     launch_pair_fill<T>(e->r_pos, e->r_off, e->nc, L, e->prm.xi, rc * rc, cut * cut, N,
-                        e->r_width, e->r_soff, e->r_col, (T *)e->r_blk, e->r_E, st);
+                        e->r_width, e->r_soff, e->r_col, (T *)e->r_blk, e->r_E, e->err, st);
   }
   HIP_TRY(hipGetLastError());
   return 0;
@@ -422,17 +424,23 @@ int build_P(_spme_t *e, hipStream_t st) {
 
 int set_positions(_spme_t *e, const double *pos, int on_device, hipStream_t st) {
   if (!pos) return fail(-1, "positions are NULL");
+  // from here on the P data and the block list are being rebuilt: until
+  // every step and check has passed the engine holds no usable positions,
+  // and groups holding it see a new version
+  e->have_pos = false;
+  ++e->pos_version;
   (void)hipGetLastError();
   HIP_TRY(hipMemcpyAsync(e->pos, pos, sizeof(double) * 3 * e->N,
                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
-  int rc = e->prec == SPME_FP32 ? build_P<float>(e, st) : build_P<double>(e, st);
+  int rc = e->tsz == 4 ? build_P<float>(e, st) : build_P<double>(e, st);
   if (rc) return rc;
   int herr = 0;
   HIP_TRY(hipMemcpyAsync(&herr, e->err, sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  if (herr) return fail(-1, "positions must be finite and within [-L, 2L)");
+  if (herr & 1) return fail(-1, "positions must be finite and within [-L, 2L)");
+  if (herr & 2)
+    return fail(-2, "real-space block list: a row found more neighbours than its count pass");
   e->have_pos = true;
-  ++e->pos_version;
   return 0;
 }
 
@@ -497,8 +505,9 @@ int check_tiles(const spme_tiles_t &t) {
   return 0;
 }
 
-template <typename T>
-int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double beta, T *out,
+// T: mesh and input type; TO: output type (double for SPME_FP32_SUM64)
+template <typename T, typename TO>
+int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double beta, TO *out,
             long ldout, int parts, hipStream_t st, int stop_after) {
   const int N = e->N, C = 3 * nrhs;
   if ((parts & SPME_SELF) && !(parts & SPME_REAL))
@@ -573,10 +582,10 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
       for (int i = fm.n; i < 5; ++i) e->pool.push_back(fm.ev[i]);
     }
     if (stop_after == 2) return 0;
-    InterpArgs<T> ia{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid,
-                     e->comp_stride, e->RS, e->K, N, C, e->P, tl.interp_zc, T(alpha), T(beta),
-                     out, ldout};
-    if (launch_interp<T>(ia, st)) return fail(-1, "interpolate: unsupported configuration");
+    InterpArgs<T, TO> ia{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid,
+                         e->comp_stride, e->RS, e->K, N, C, e->P, tl.interp_zc, TO(alpha),
+                         TO(beta), out, ldout};
+    if (launch_interp(ia, st)) return fail(-1, "interpolate: unsupported configuration");
     mark(SPME_ST_INTERP);
   }
   if (parts & SPME_REAL) {
@@ -589,10 +598,10 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
                             ? 1.0 - 6.0 * xi / std::sqrt(M_PI) +
                                   40.0 * xi * xi * xi / (3.0 * std::sqrt(M_PI))
                             : 0.0;
-    RealArgs<T> ra{e->r_perm, e->r_width, e->r_soff, e->r_col, (const T *)e->r_blk, e->r_E,
+    RealArgs<T, TO> ra{e->r_perm, e->r_width, e->r_soff, e->r_col, (const T *)e->r_blk, e->r_E,
                    e->r_pos, e->prm.box_size, self, (const T *)e->r_fs, N, C, alpha, beta,
                    (parts & SPME_RECIP) ? 1 : 0, out, ldout};
-    if (launch_real<T>(ra, st)) return fail(-1, "real space: unsupported batch width");
+    if (launch_real(ra, st)) return fail(-1, "real space: unsupported batch width");
     mark(SPME_ST_REAL);
   }
   HIP_TRY(hipGetLastError());
@@ -611,10 +620,13 @@ int apply(_spme_t *e, int nrhs, double alpha, const void *in, long ldin, double 
   if (ldin < 3L * e->N || ldout < 3L * e->N) return fail(-1, "ldin/ldout must be >= 3*np");
   if (!in || !out) return fail(-1, "null vector");
   if (parts == 0 || (parts & ~SPME_FULL)) return fail(-1, "bad parts mask %d", parts);
+  if (e->prec == SPME_FP32_SUM64)
+    return apply_t<float, double>(e, nrhs, alpha, (const float *)in, ldin, beta, (double *)out,
+                                  ldout, parts, st, stop_after);
   if (e->prec == SPME_FP32)
-    return apply_t<float>(e, nrhs, alpha, (const float *)in, ldin, beta, (float *)out, ldout,
+    return apply_t<float, float>(e, nrhs, alpha, (const float *)in, ldin, beta, (float *)out, ldout,
                           parts, st, stop_after);
-  return apply_t<double>(e, nrhs, alpha, (const double *)in, ldin, beta, (double *)out, ldout,
+  return apply_t<double, double>(e, nrhs, alpha, (const double *)in, ldin, beta, (double *)out, ldout,
                          parts, st, stop_after);
 }
 
@@ -669,13 +681,13 @@ std::vector<int> zc_candidates(int K, int lo, int hi) {
   return v;
 }
 
-template <typename T>
+template <typename T, typename TO>
 int autotune_t(_spme_t *e, int nrhs, int reps, hipStream_t st) {
   const int C = 3 * nrhs, K = e->K, N = e->N;
   EventPair ev;
   if (int rc = ev.init()) return rc;
   DevBuf out;
-  HIP_TRY(hipMalloc(&out.p, sizeof(T) * (size_t)C * N));
+  HIP_TRY(hipMalloc(&out.p, sizeof(TO) * (size_t)C * N));
   HIP_TRY(hipMemsetAsync(e->fs, 0, e->tsz * (size_t)C * N, st));
   spme_tiles_t best = e->tiles;
   float best_ms = 1e30f;
@@ -696,11 +708,11 @@ int autotune_t(_spme_t *e, int nrhs, int reps, hipStream_t st) {
   }
   best_ms = 1e30f;
   for (int zc : zc_candidates(K, 8, std::min(K, (int)InterpShape<T>::MAXZC))) {
-    InterpArgs<T> ia{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid,
-                     e->comp_stride, e->RS, K, N, C, e->P, zc, T(1), T(0), (T *)out.p,
-                     3L * N};
+    InterpArgs<T, TO> ia{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid,
+                         e->comp_stride, e->RS, K, N, C, e->P, zc, TO(1), TO(0), (TO *)out.p,
+                         3L * N};
     float ms = 0.f;
-    if (int rc = time_launch(ev, st, reps, [&] { return launch_interp<T>(ia, st); }, ms))
+    if (int rc = time_launch(ev, st, reps, [&] { return launch_interp(ia, st); }, ms))
       return rc;
     if (ms < best_ms) {
       best_ms = ms;
@@ -1260,8 +1272,9 @@ int spme_engine_autotune(spme_t *spme, int nrhs, int reps, spme_tiles_t *best, v
     return fail(-1, "nrhs must be in [1, %d] (got %d)", spme->max_rhs, nrhs);
   if (reps < 1) return fail(-1, "reps must be positive");
   hipStream_t st = (hipStream_t)stream;
-  const int rc = spme->prec == SPME_FP32 ? autotune_t<float>(spme, nrhs, reps, st)
-                                         : autotune_t<double>(spme, nrhs, reps, st);
+  const int rc = spme->prec == SPME_FP32 ? autotune_t<float, float>(spme, nrhs, reps, st)
+                 : spme->prec == SPME_FP32_SUM64 ? autotune_t<float, double>(spme, nrhs, reps, st)
+                                                 : autotune_t<double, double>(spme, nrhs, reps, st);
   if (rc) return rc;
   return best ? spme_engine_get_tiles(spme, nrhs, best) : 0;
 }
@@ -1274,6 +1287,8 @@ int spme_group_create(int nsys, spme_t *const *engines, int nrhs, spme_group_t *
     const spme_t *e = engines[s];
     if (!e) return fail(-1, "engine %d is null", s);
     if (e->dense) return fail(-1, "engine %d: plain-path engines cannot be grouped", s);
+    if (e->prec == SPME_FP32_SUM64)
+      return fail(-1, "engine %d: groups take SPME_FP32 or SPME_FP64 engines", s);
     if (e->prec != engines[0]->prec || e->P != engines[0]->P)
       return fail(-1, "engine %d: all engines need one precision and one porder", s);
     if (nrhs < 1 || nrhs > e->max_rhs)

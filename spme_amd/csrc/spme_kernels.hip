@@ -225,17 +225,17 @@ __global__ __launch_bounds__(64) void k_spread(const int *__restrict__ boff,
 }
 
 // Interpolation: single-system kernel.
-template <typename T, int P, int CC, int TXY, int NT>
+template <typename T, int P, int CC, int TXY, int NT, typename TO>
 __global__ __launch_bounds__(NT) void k_interp(const int *__restrict__ boff,
                                                 const int *__restrict__ pbase,
                                                 const T *__restrict__ pw,
                                                 const int *__restrict__ perm,
                                                 const T *__restrict__ grid, long comp_stride,
-                                                int RS, int K, int C, int ZC, T alpha, T beta,
-                                                T *__restrict__ out, long ldout) {
+                                                int RS, int K, int C, int ZC, TO alpha, TO beta,
+                                                TO *__restrict__ out, long ldout) {
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
-  interp_block<T, P, CC, TXY, NT>(boff, pbase, pw, perm, grid, comp_stride, RS, K, C, ZC, alpha,
-                                  beta, out, ldout, lb);
+  interp_block<T, P, CC, TXY, NT, TO>(boff, pbase, pw, perm, grid, comp_stride, RS, K, C, ZC,
+                                      alpha, beta, out, ldout, lb);
 }
 
 
@@ -824,17 +824,17 @@ int launch_spread(const SpreadArgs<T> &a, hipStream_t st) {
 
 // Interpolation shape: 16 bytes of components (4 FP32, 2 FP64) per block
 // over a 16 x 16 tile (8 x 8 when K is not a multiple of 16).
-template <typename T, int P, int CC, int TXY>
-static void interp_cc(const InterpArgs<T> &a, hipStream_t st) {
+template <typename T, int P, int CC, int TXY, typename TO>
+static void interp_cc(const InterpArgs<T, TO> &a, hipStream_t st) {
   constexpr int NT = InterpShape<T>::NT;
   const int nblk = interp_blocks(a.K, a.ZC, a.C, CC, TXY);
-  hipLaunchKernelGGL((k_interp<T, P, CC, TXY, NT>), dim3(nblk), dim3(NT), 0, st, a.boff, a.pbase,
-                     a.pw, a.perm, a.grid, a.comp_stride, a.RS, a.K, a.C, a.ZC, a.alpha, a.beta,
-                     a.out, a.ldout);
+  hipLaunchKernelGGL((k_interp<T, P, CC, TXY, NT, TO>), dim3(nblk), dim3(NT), 0, st, a.boff,
+                     a.pbase, a.pw, a.perm, a.grid, a.comp_stride, a.RS, a.K, a.C, a.ZC, a.alpha,
+                     a.beta, a.out, a.ldout);
 }
 
-template <typename T, int P>
-static int interp_p(const InterpArgs<T> &a, hipStream_t st) {
+template <typename T, int P, typename TO>
+static int interp_p(const InterpArgs<T, TO> &a, hipStream_t st) {
   constexpr int CC = InterpShape<T>::CC;
   if (a.ZC < 1 || a.ZC > InterpShape<T>::MAXZC) return -1;
   if (a.K % 16 == 0) interp_cc<T, P, CC, 16>(a, st);
@@ -842,8 +842,8 @@ static int interp_p(const InterpArgs<T> &a, hipStream_t st) {
   return 0;
 }
 
-template <typename T>
-int launch_interp(const InterpArgs<T> &a, hipStream_t st) {
+template <typename T, typename TO>
+int launch_interp(const InterpArgs<T, TO> &a, hipStream_t st) {
   switch (a.P) {
     case 4: return interp_p<T, 4>(a, st);
     case 6: return interp_p<T, 6>(a, st);
@@ -1009,8 +1009,9 @@ template void launch_permute_in<double>(const int *, double *, const int *, doub
                                         const double *, long, int, int, hipStream_t);
 template int launch_spread<float>(const SpreadArgs<float> &, hipStream_t);
 template int launch_spread<double>(const SpreadArgs<double> &, hipStream_t);
-template int launch_interp<float>(const InterpArgs<float> &, hipStream_t);
-template int launch_interp<double>(const InterpArgs<double> &, hipStream_t);
+template int launch_interp<float, float>(const InterpArgs<float, float> &, hipStream_t);
+template int launch_interp<float, double>(const InterpArgs<float, double> &, hipStream_t);
+template int launch_interp<double, double>(const InterpArgs<double, double> &, hipStream_t);
 template int launch_fft_influence<float>(const FftArgs<float> &, hipStream_t, FftMarks *);
 template int launch_fft_influence<double>(const FftArgs<double> &, hipStream_t, FftMarks *);
 

@@ -24,7 +24,9 @@ template <typename T> struct SpreadArgs {
   int RS, K, ZC;
 };
 
-template <typename T> struct InterpArgs {
+// TO: type of the output vectors and of the final beta*out + alpha*sum
+// (T, or double when an FP32 mesh path forms its final sum in FP64).
+template <typename T, typename TO = T> struct InterpArgs {
   const int *boff;
   const int *pbase;
   const T *pw;
@@ -32,8 +34,8 @@ template <typename T> struct InterpArgs {
   const T *grid;
   long comp_stride;
   int RS, K, N, C, P, ZC;
-  T alpha, beta;
-  T *out;
+  TO alpha, beta;
+  TO *out;
   long ldout;
 };
 
@@ -138,7 +140,7 @@ void launch_permute_in(const int *ia, T *fa, const int *ib, T *fb, const T *in, 
                        int C, hipStream_t st);
 void launch_invert(const int *perm, int *inv, int N, hipStream_t st);
 template <typename T> int launch_spread(const SpreadArgs<T> &a, hipStream_t st);
-template <typename T> int launch_interp(const InterpArgs<T> &a, hipStream_t st);
+template <typename T, typename TO> int launch_interp(const InterpArgs<T, TO> &a, hipStream_t st);
 // mk (optional): events to record after each launch (mk->n = 0 on entry).
 template <typename T>
 int launch_fft_influence(const FftArgs<T> &a, hipStream_t st, FftMarks *mk = nullptr);

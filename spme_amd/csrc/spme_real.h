@@ -4,7 +4,8 @@
 
 namespace spme {
 
-template <typename T> struct RealArgs {
+// TO: type of the output vectors and of the final sum (see InterpArgs).
+template <typename T, typename TO = T> struct RealArgs {
   const int *perm;   // cell order -> original index
   const int *width;  // entries per row, per slice of 64 rows
   const int *soff;   // slice offsets (entries)
@@ -18,7 +19,7 @@ template <typename T> struct RealArgs {
   int N, C;
   double alpha, beta;
   int accum;
-  T *out;
+  TO *out;
   long ldout;
 };
 
@@ -32,7 +33,7 @@ void launch_slice_width(const int *cnt, int N, int *width, int *size, hipStream_
 template <typename T>
 void launch_pair_fill(const double *spos, const int *coff, int nc, double L, double xi,
                       double rc2, double cut2, int N, const int *width, const int *soff,
-                      int *col, T *blk, long E, hipStream_t st);
-template <typename T> int launch_real(const RealArgs<T> &a, hipStream_t st);
+                      int *col, T *blk, long E, int *err, hipStream_t st);
+template <typename T, typename TO> int launch_real(const RealArgs<T, TO> &a, hipStream_t st);
 
 }  // namespace spme

@@ -155,22 +155,29 @@ __global__ __launch_bounds__(64) void k_slice_width(const int *__restrict__ cnt,
 // own index.  The coefficients follow in a second pass over the stored
 // entries only: computed here, the FP64 erfc/exp of a hit would hold up the
 // whole wave on every candidate that one lane accepts (about one in seven).
+// The slice widths come from k_pair_count's decisions; a row that finds
+// more neighbours here than its slice holds (the two passes disagreeing on
+// a distance test) stores none past the width and sets bit 2 of err.
 __global__ __launch_bounds__(256) void k_pair_cols(const double *__restrict__ spos,
                                                    const int *__restrict__ coff, int nc,
                                                    double L, double cut2, int N,
                                                    const int *__restrict__ width,
                                                    const int *__restrict__ soff,
-                                                   int *__restrict__ col) {
+                                                   int *__restrict__ col, int *__restrict__ err) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   const int q = s >> 6, l = s & 63;
   const long base = soff[q] + l;
+  const int w = width[q];
   int k = 0;
+  bool over = false;
   for_neighbours(spos, coff, nc, L, s, cut2, [&](int j, double, double, double, double) {
-    col[base + 64L * k] = j;
+    if (k < w) col[base + 64L * k] = j;
+    else over = true;
     ++k;
   });
-  for (; k < width[q]; ++k) col[base + 64L * k] = s;
+  if (over) atomicOr(err, 2);
+  for (; k < w; ++k) col[base + 64L * k] = s;
 }
 
 // F and G/r^2 of every stored entry (zero for the padding on the row itself).
@@ -206,7 +213,7 @@ __global__ __launch_bounds__(256) void k_pair_coeffs(const double *__restrict__ 
 }
 
 // Lanes = (row in cell order, right-hand side), right-hand side fastest.
-template <typename T, int C>
+template <typename T, int C, typename TO>
 __global__ __launch_bounds__(256) void k_real_spmv(const int *__restrict__ perm,
                                                    const int *__restrict__ width,
                                                    const int *__restrict__ soff,
@@ -214,14 +221,14 @@ __global__ __launch_bounds__(256) void k_real_spmv(const int *__restrict__ perm,
                                                    const T *__restrict__ blk, long E,
                                                    const double *__restrict__ spos, double L,
                                                    T self, const T *__restrict__ fs, int N,
-                                                   T alpha, T beta, int accum,
-                                                   T *__restrict__ out, long ldout) {
+                                                   TO alpha, TO beta, int accum,
+                                                   TO *__restrict__ out, long ldout) {
   constexpr int LPR = RealShape<C>::LPR;
   const long g = (long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int s = int(g / LPR), l = int(g % LPR);
   if (s < N)
-    real_lane<T, C>(perm, width, soff, col, blk, E, spos, L, self, fs, alpha, beta, accum, out,
-                    ldout, s, l);
+    real_lane<T, C, TO>(perm, width, soff, col, blk, E, spos, L, self, fs, alpha, beta, accum,
+                        out, ldout, s, l);
 }
 
 static int grid1(long n, int b) { return int((n + b - 1) / b); }
@@ -250,28 +257,25 @@ void launch_slice_width(const int *cnt, int N, int *width, int *size, hipStream_
 template <typename T>
 void launch_pair_fill(const double *spos, const int *coff, int nc, double L, double xi,
                       double rc2, double cut2, int N, const int *width, const int *soff,
-                      int *col, T *blk, long E, hipStream_t st) {
+                      int *col, T *blk, long E, int *err, hipStream_t st) {
   hipLaunchKernelGGL(k_pair_cols, dim3(grid1(N, 256)), dim3(256), 0, st, spos, coff, nc, L,
-                     cut2, N, width, soff, col);
+                     cut2, N, width, soff, col, err);
   hipLaunchKernelGGL(k_pair_coeffs<T>, dim3(grid1(N, 256)), dim3(256), 0, st, spos, N, L, xi,
                      rc2, width, soff, col, blk, E);
 }
 
-template <typename T, int C>
-static void real_c(const RealArgs<T> &a, hipStream_t st) {
-  hipLaunchKernelGGL((k_real_spmv<T, C>), dim3(grid1((long)a.N * RealShape<C>::LPR, 256)),
-                     dim3(256), 0,
-                     st, a.perm,
-                     a.width, a.soff, a.col, a.blk, a.E, a.spos, a.L, T(a.self), a.fs, a.N,
-                     T(a.alpha),
-                     T(a.beta), a.accum, a.out, a.ldout);
+template <typename T, int C, typename TO>
+static void real_c(const RealArgs<T, TO> &a, hipStream_t st) {
+  hipLaunchKernelGGL((k_real_spmv<T, C, TO>), dim3(grid1((long)a.N * RealShape<C>::LPR, 256)),
+                     dim3(256), 0, st, a.perm, a.width, a.soff, a.col, a.blk, a.E, a.spos, a.L,
+                     T(a.self), a.fs, a.N, TO(a.alpha), TO(a.beta), a.accum, a.out, a.ldout);
 }
 
-template <typename T>
-int launch_real(const RealArgs<T> &a, hipStream_t st) {
+template <typename T, typename TO>
+int launch_real(const RealArgs<T, TO> &a, hipStream_t st) {
   switch (a.C) {  // every batch width 1..16
 #define X(CC) \
-  case CC: real_c<T, CC>(a, st); return 0;
+  case CC: real_c<T, CC, TO>(a, st); return 0;
     X(3) X(6) X(9) X(12) X(15) X(18) X(21) X(24) X(27) X(30) X(33) X(36) X(39) X(42) X(45)
     X(48)
 #undef X
@@ -281,11 +285,12 @@ int launch_real(const RealArgs<T> &a, hipStream_t st) {
 
 template void launch_pair_fill<float>(const double *, const int *, int, double, double, double,
                                       double, int, const int *, const int *, int *, float *,
-                                      long, hipStream_t);
+                                      long, int *, hipStream_t);
 template void launch_pair_fill<double>(const double *, const int *, int, double, double, double,
                                        double, int, const int *, const int *, int *, double *,
-                                       long, hipStream_t);
-template int launch_real<float>(const RealArgs<float> &, hipStream_t);
-template int launch_real<double>(const RealArgs<double> &, hipStream_t);
+                                       long, int *, hipStream_t);
+template int launch_real<float, float>(const RealArgs<float, float> &, hipStream_t);
+template int launch_real<float, double>(const RealArgs<float, double> &, hipStream_t);
+template int launch_real<double, double>(const RealArgs<double, double> &, hipStream_t);
 
 }  // namespace spme

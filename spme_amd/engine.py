@@ -16,11 +16,11 @@ import ctypes
 import numpy as np
 
 from . import lib as _L
-from .lib import (SPME_FP32, SPME_FP64, SPME_FP64_PLAIN, SPME_FULL, SPME_REAL,
+from .lib import (SPME_FP32, SPME_FP32_SUM64, SPME_FP64, SPME_FP64_PLAIN, SPME_FULL, SPME_REAL,
                   SPME_RECIP, SPME_SELF, SpmeError, SpmeInfo, SpmeParams, SpmeTiles,
                   check, load)
 
-__all__ = ["Engine", "Group", "DropIn", "cutoff2xi", "SPME_FP32", "SPME_FP64",
+__all__ = ["Engine", "Group", "DropIn", "cutoff2xi", "SPME_FP32", "SPME_FP32_SUM64", "SPME_FP64",
            "SPME_FP64_PLAIN", "SPME_FULL", "SPME_REAL", "SPME_RECIP", "SPME_SELF",
            "SpmeError"]
 
@@ -59,6 +59,14 @@ class Engine:
 
     @property
     def dtype(self):
+        """torch dtype of the input vectors (and of the mesh path)."""
+        torch = _torch()
+        return torch.float32 if self.precision in (SPME_FP32, SPME_FP32_SUM64) else torch.float64
+
+    @property
+    def out_dtype(self):
+        """torch dtype of the output vectors: double for SPME_FP32_SUM64,
+        whose final real + reciprocal sum is formed in FP64."""
         torch = _torch()
         return torch.float32 if self.precision == SPME_FP32 else torch.float64
 
@@ -109,10 +117,10 @@ class Engine:
         if u is not None and u.dim() == 1:
             u = u.view(1, -1)
         if u is None:
-            u = torch.zeros_like(f)
+            u = torch.zeros(f.shape, dtype=self.out_dtype, device=f.device)
             beta = 0.0
         nrhs = f.shape[0]
-        assert f.dtype == self.dtype and u.dtype == self.dtype
+        assert f.dtype == self.dtype and u.dtype == self.out_dtype
         assert f.stride(1) == 1 and u.stride(1) == 1
         s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
         check(self._lib.spme_engine_apply(
@@ -161,7 +169,7 @@ class Engine:
         inf = self.info()
         f = f.view(1, -1) if f.dim() == 1 else f
         nrhs = f.shape[0]
-        dt = np.float32 if self.precision == SPME_FP32 else np.float64
+        dt = np.float32 if self.precision in (SPME_FP32, SPME_FP32_SUM64) else np.float64
         out = np.empty(3 * nrhs * inf.comp_stride, dtype=dt)
         check(self._lib.spme_engine_debug_stage(
             self._h, int(stage), int(nrhs), ctypes.c_void_p(f.data_ptr()),

@@ -271,6 +271,9 @@ def engine_lanczos(engine, s: int, tol: float = 1e-6, max_iter: int = 100,
     """Block Lanczos over an engine's batched apply (full operator)."""
     if s > engine.max_rhs:
         raise SpmeError(f"block width {s} exceeds the engine's max_rhs {engine.max_rhs}")
+    if engine.dtype != engine.out_dtype:
+        raise SpmeError("block Lanczos needs an engine whose inputs and outputs share one "
+                        "precision (SPME_FP32 or SPME_FP64)")
     n = 3 * engine.info().N
     ops = HipBlockOps(engine.precision, n)
 

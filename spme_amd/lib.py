@@ -12,7 +12,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libspme_hip.so")
 
-SPME_FP64, SPME_FP32, SPME_FP64_PLAIN = 0, 1, 2
+SPME_FP64, SPME_FP32, SPME_FP64_PLAIN, SPME_FP32_SUM64 = 0, 1, 2, 3
 SPME_RECIP, SPME_REAL, SPME_SELF, SPME_FULL = 1, 2, 4, 7
 
 # Every symbol the public headers declare.

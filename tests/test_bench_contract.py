@@ -39,3 +39,17 @@ def test_more_than_one_gpu_is_refused_with_a_reason():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in msg["error"]
+
+
+def test_fp64_sum_form_counts_double_outputs():
+    """With the final sum in FP64 the output vectors are 8-byte values: the
+    interpolation writes 3NR more 4-byte words and the real-space pass reads
+    and writes 2 x 3NR more."""
+    sys.path.insert(0, ROOT)
+    import bench
+    N, K, p, R = 200_000, 192, 8, 16
+    a = bench.algorithmic_bytes(N, K, p, R, 4, 1000)
+    b = bench.algorithmic_bytes(N, K, p, R, 4, 1000, so=8)
+    assert b["interp"] - a["interp"] == 3 * N * R * 4
+    assert b["real"] - a["real"] == 2 * 3 * N * R * 4
+    assert b["spread"] == a["spread"] and b["fft_total"] == a["fft_total"]

@@ -134,3 +134,34 @@ def test_extreme_clustering(oracle, E, torch):
     with pytest.raises(SpmeError, match="too clustered"):
         eng.set_positions(torch.tensor(pos, device="cuda"))
     eng.close()
+
+
+def test_failed_set_positions_leaves_no_positions(E, torch):
+    """A set_positions that fails after an earlier success leaves the engine
+    without positions (not with half-rebuilt P data and block list): good
+    positions, then a set refused as too clustered, then one with a NaN;
+    each refusal is followed by an apply, which must report that positions
+    have not been set, and good positions make it usable again."""
+    N, L, K, p, xi = 50_000, 40.0, 32, 6, 0.5
+    good = W.positions(81, N, L)
+    clustered = 20.0 + np.random.default_rng(12).uniform(-0.5, 0.5, (N, 3))
+    f = torch.tensor(W.forces(82, N, 1), device="cuda")
+    _say("case: good positions, then refused ones, then apply")
+    eng = E.Engine(xi, K, p, N, L, rc=5.0, precision=E.SPME_FP64)
+    eng.set_positions(torch.tensor(good, device="cuda"))
+    u_good = eng.apply(f, parts=E.SPME_FULL).cpu().numpy()
+    with pytest.raises(SpmeError, match="too clustered"):
+        eng.set_positions(torch.tensor(clustered, device="cuda"))
+    for parts in (E.SPME_FULL, E.SPME_RECIP):
+        with pytest.raises(SpmeError, match="positions have not been set"):
+            eng.apply(f, parts=parts)
+    bad = good.copy()
+    bad[17, 2] = np.nan
+    eng.set_positions(torch.tensor(good, device="cuda"))
+    with pytest.raises(SpmeError, match="finite"):
+        eng.set_positions(torch.tensor(bad, device="cuda"))
+    with pytest.raises(SpmeError, match="positions have not been set"):
+        eng.apply(f, parts=E.SPME_FULL)
+    eng.set_positions(torch.tensor(good, device="cuda"))
+    np.testing.assert_array_equal(eng.apply(f, parts=E.SPME_FULL).cpu().numpy(), u_good)
+    eng.close()

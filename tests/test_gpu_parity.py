@@ -294,6 +294,47 @@ def test_full_mobility_vs_oracle_and_direct(oracle, E, torch):
         assert err < acc, (prec, err)
 
 
+def test_fp32_sum64_form(oracle, E, torch):
+    """SPME_FP32_SUM64: the FP32 mesh path whose final beta*u + alpha*(recip +
+    real) sum is formed in FP64 (float inputs, double outputs).  Against the
+    oracle at the FP32 bar, with alpha and beta; against the FP32 engine the
+    only difference is the final rounding (within 2e-7 relative); and the
+    outputs carry bits beyond FP32 (most entries), so the sum was not
+    rounded to float."""
+    _say("case: fp32 mesh path with an fp64 final sum, full M.f, alpha/beta")
+    N, K, p, rc, xi = 1000, 64, 8, 10.0, 0.5
+    pos, L = _system(1001, N, 0.1)
+    f = W.forces(1001, N, 2).astype(np.float32).astype(np.float64)
+    u0 = W.forces(1005, N, 2)
+    ref = oracle.spme_full(xi, K, p, rc, L, pos, f)
+    eng = E.Engine(xi, K, p, N, L, rc=rc, precision=E.SPME_FP32_SUM64, max_rhs=2)
+    eng.set_positions(pos)
+    f32 = torch.tensor(f, device="cuda", dtype=torch.float32)
+    u = eng.apply(f32, parts=E.SPME_FULL)
+    assert u.dtype == torch.float64
+    got = u.cpu().numpy()
+    assert _rel(got, ref) < FP32_TOL
+    ub = torch.tensor(u0, device="cuda")
+    eng.apply(f32, ub, alpha=-0.7, beta=1.9, parts=E.SPME_FULL)
+    assert _rel(ub.cpu().numpy(), 1.9 * u0 - 0.7 * ref) < FP32_TOL
+    # beta*u0 keeps FP64 precision: the result minus the FP64 reference part
+    # is the FP32 operator's error only
+    assert np.abs(ub.cpu().numpy() - (1.9 * u0 - 0.7 * got)).max() < 1e-12
+    # the sum of the two FP32 parts is itself an FP32 value when their
+    # exponents are close (70 % of entries here are not)
+    assert np.mean(got != got.astype(np.float32).astype(np.float64)) > 0.5
+    eng.close()
+    e32 = E.Engine(xi, K, p, N, L, rc=rc, precision=E.SPME_FP32, max_rhs=2)
+    e32.set_positions(pos)
+    g32 = e32.apply(f32, parts=E.SPME_FULL).double().cpu().numpy()
+    e32.close()
+    assert _rel(g32, got) < 2e-7
+    with pytest.raises(E.SpmeError, match="groups take"):
+        g = E.Engine(xi, K, p, N, L, rc=rc, precision=E.SPME_FP32_SUM64, max_rhs=2)
+        g.set_positions(pos)
+        E.Group([g], 2)
+
+
 def test_deterministic(E, torch):
     _say("case: run twice bitwise, fp32 K=48 p=6 R=4")
     N, K, p = 3000, 48, 6
