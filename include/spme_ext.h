@@ -192,6 +192,21 @@ int spme_block_gram(int precision, long n, int sx, int sy, const void *X, long l
 int spme_block_update(int precision, long n, int sx, int sy, const void *X, long ldx,
                       const double *A, double beta, void *Y, long ldy, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): spme_block_gram without the workspace argument, calling
+ * no allocator.  The partial sums live in a static device array of
+ * spme_block_gram_noalloc_max() doubles (8 MiB) that each GPU holds until
+ * the process ends.  Where spme_block_gram's partial sums fit in it the
+ * partition, and so the result, is bitwise the same; larger tall products
+ * (sx > 16) use fewer blocks along n, a deterministic partition of their
+ * own.  Sizes beyond the block limits above are refused with a message.
+ * All calls on one GPU share the array: calls that may overlap on one GPU
+ * (other streams, other threads) race. */
+long spme_block_gram_noalloc_max(void);
+int spme_block_gram_noalloc(int precision, long n, int sx, int sy, const void *X, long ldx,
+                            const void *Y, long ldy, double *C, void *stream);
+
 /* Layout and sizes, for tests and benchmarks. */
 typedef struct spme_info {
   int K, P, N, precision, max_rhs;

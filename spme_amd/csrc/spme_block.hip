@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <mutex>
 
 #include "../../include/spme_ext.h"
 #include "spme_launch.h"
@@ -246,6 +247,37 @@ __global__ __launch_bounds__(256) void k_block_update_tall(long n, int sx, int s
   }
 }
 
+// Launches the partial-sum kernel over nbx blocks along n (tall form: nbx
+// blocks per group of TA rows, 4 partial sums each) and the fixed-order
+// reduction of the partial sums in work into C.
+int gram_launch(int precision, long n, int sx, int sy, const void *X, long ldx, const void *Y,
+                long ldy, double *C, double *work, int nbx, hipStream_t st) {
+  int nb = nbx;
+  if (sx <= BS) {
+    if (precision == SPME_FP32)
+      hipLaunchKernelGGL(k_block_gram_part<float>, dim3(nbx), dim3(256), 0, st, n, sx, sy,
+                         (const float *)X, ldx, (const float *)Y, ldy, work);
+    else
+      hipLaunchKernelGGL(k_block_gram_part<double>, dim3(nbx), dim3(256), 0, st, n, sx, sy,
+                         (const double *)X, ldx, (const double *)Y, ldy, work);
+  } else {
+    nb = 4 * nbx;
+    const dim3 grid(nbx, (sx + TA - 1) / TA);
+    if (precision == SPME_FP32)
+      hipLaunchKernelGGL(k_block_gram_tall<float>, grid, dim3(256), 0, st, n, sx, sy,
+                         (const float *)X, ldx, (const float *)Y, ldy, work);
+    else
+      hipLaunchKernelGGL(k_block_gram_tall<double>, grid, dim3(256), 0, st, n, sx, sy,
+                         (const double *)X, ldx, (const double *)Y, ldy, work);
+  }
+  hipLaunchKernelGGL(k_block_gram_reduce, dim3(sx * sy), dim3(64), 0, st, nb, sx * sy, work, C);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : fail(-2, "block gram launch: %s", hipGetErrorString(e));
+}
+
+// Blocks along n of the workspace form: gram_parts partial sums.
+int gram_nbx(long n, int sx) { return sx <= BS ? gram_blocks(n) : gram_blocks_tall(n, sx); }
+
 int check_block(int precision, long n, int sx, int sy, long ldx, long ldy) {
   if (precision != SPME_FP32 && precision != SPME_FP64 && precision != SPME_FP64_PLAIN)
     return fail(-1, "precision must be SPME_FP32, SPME_FP64 or SPME_FP64_PLAIN");
@@ -255,6 +287,46 @@ int check_block(int precision, long n, int sx, int sy, long ldx, long ldy) {
   if (ldx < n || ldy < n) return fail(-1, "leading dimensions must be >= n");
   return 0;
 }
+// ---------------------------------------------------------------------------
+// The same product without a workspace argument, for callers that cannot
+// allocate: the partial sums live in this file-scope device array, one copy
+// per GPU, held until the process ends.  Host code must not pass the
+// array's name to a kernel (on the host it is only a stand-in for the
+// symbol): its device address is looked up once per device with
+// hipGetSymbolAddress, on that device's first call, and used from then on.
+// Every partial sum a launch reduces is written by that launch, so nothing
+// needs clearing between calls.  All calls on one GPU share the array:
+// calls that may overlap on one GPU (on different streams, or from several
+// threads without ordering) race.
+constexpr long GRAM_SCRATCH = 1L << 20;  // doubles (8 MiB)
+static __device__ __align__(256) double scratch[GRAM_SCRATCH];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of this device's scratch array (nullptr and an error message
+// on failure).
+double *scratch_on_current_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) {
+    fail(-2, "no usable current device for the no-alloc gram");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      fail(-2, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+           hipGetErrorString(e));
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = static_cast<double *>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+
 }  // namespace
 }  // namespace spme
 
@@ -275,27 +347,31 @@ int spme_block_gram(int precision, long n, int sx, int sy, const void *X, long l
   const long need = spme_block_gram_worksize(n, sx, sy);
   if (work_len < need) return fail(-1, "workspace too small (%ld < %ld doubles)", work_len, need);
   (void)hipGetLastError();
-  hipStream_t st = (hipStream_t)stream;
-  const int nb = gram_parts(n, sx);
-  if (sx <= BS) {
-    if (precision == SPME_FP32)
-      hipLaunchKernelGGL(k_block_gram_part<float>, dim3(nb), dim3(256), 0, st, n, sx, sy,
-                         (const float *)X, ldx, (const float *)Y, ldy, work);
-    else
-      hipLaunchKernelGGL(k_block_gram_part<double>, dim3(nb), dim3(256), 0, st, n, sx, sy,
-                         (const double *)X, ldx, (const double *)Y, ldy, work);
-  } else {
-    const dim3 grid(nb / 4, (sx + TA - 1) / TA);
-    if (precision == SPME_FP32)
-      hipLaunchKernelGGL(k_block_gram_tall<float>, grid, dim3(256), 0, st, n, sx, sy,
-                         (const float *)X, ldx, (const float *)Y, ldy, work);
-    else
-      hipLaunchKernelGGL(k_block_gram_tall<double>, grid, dim3(256), 0, st, n, sx, sy,
-                         (const double *)X, ldx, (const double *)Y, ldy, work);
-  }
-  hipLaunchKernelGGL(k_block_gram_reduce, dim3(sx * sy), dim3(64), 0, st, nb, sx * sy, work, C);
-  const hipError_t e = hipGetLastError();
-  return e == hipSuccess ? 0 : fail(-2, "block gram launch: %s", hipGetErrorString(e));
+  return gram_launch(precision, n, sx, sy, X, ldx, Y, ldy, C, work, gram_nbx(n, sx),
+                     (hipStream_t)stream);
+}
+
+
+long spme_block_gram_noalloc_max(void) { return GRAM_SCRATCH; }
+
+int spme_block_gram_noalloc(int precision, long n, int sx, int sy, const void *X, long ldx,
+                            const void *Y, long ldy, double *C, void *stream) {
+  if (int rc = check_block(precision, n, sx, sy, ldx, ldy)) return rc;
+  if (!X || !Y || !C) return fail(-1, "null pointer");
+  // the workspace form's partition when its partial sums fit, so the result
+  // is bitwise that of spme_block_gram; otherwise fewer blocks along n (the
+  // tall form only: a partition of its own, fixed by n and sx, so the result
+  // stays deterministic)
+  int nbx = gram_nbx(n, sx);
+  const long per = sx <= BS ? (long)sx * sy : 4L * sx * sy;  // doubles per block along n
+  if ((long)nbx * per > GRAM_SCRATCH) nbx = int(GRAM_SCRATCH / per);
+  if (nbx < 1)
+    return fail(-1, "no-alloc gram: %d x %d partial sums exceed the %ld-double scratch array",
+                sx, sy, GRAM_SCRATCH);
+  double *work = scratch_on_current_device();
+  if (!work) return -2;
+  (void)hipGetLastError();
+  return gram_launch(precision, n, sx, sy, X, ldx, Y, ldy, C, work, nbx, (hipStream_t)stream);
 }
 
 int spme_block_update(int precision, long n, int sx, int sy, const void *X, long ldx,

@@ -26,6 +26,7 @@ EXPORTED = [
     "spme_group_create", "spme_group_destroy", "spme_group_set_tiles",
     "spme_group_get_tiles", "spme_group_autotune", "spme_group_apply",
     "spme_block_gram_worksize", "spme_block_gram", "spme_block_update",
+    "spme_block_gram_noalloc_max", "spme_block_gram_noalloc",
 ]
 
 TILE_FIELDS = ["spread_cc", "spread_zc", "interp_zc", "spread_persist",
@@ -116,6 +117,9 @@ def load() -> ctypes.CDLL:
     L.spme_block_gram_worksize.restype = l
     L.spme_block_gram.argtypes = [i, l, i, i, vp, l, vp, l, vp, vp, l, vp]
     L.spme_block_update.argtypes = [i, l, i, i, vp, l, vp, d, vp, l, vp]
+    L.spme_block_gram_noalloc_max.argtypes = []
+    L.spme_block_gram_noalloc_max.restype = l
+    L.spme_block_gram_noalloc.argtypes = [i, l, i, i, vp, l, vp, l, vp, vp]
     _lib = L
     return L
 
