@@ -989,8 +989,8 @@ template <int C> struct RealShape {
   static constexpr int LPR = real_lpr(C);  // lanes per row
 };
 
-// N values from p into v; 16-byte loads when N values fill whole 16-byte
-// units (p is then 16-byte aligned at every call site).
+// N values from p into v; 16-byte (8-byte) loads when N values fill whole
+// 16-byte (8-byte) units (p is then so aligned at every call site).
 template <typename T, int N>
 __device__ __forceinline__ void load_n(const T *__restrict__ p, T (&v)[N]) {
   if constexpr (N * sizeof(T) % 16 == 0) {
@@ -1001,6 +1001,14 @@ __device__ __forceinline__ void load_n(const T *__restrict__ p, T (&v)[N]) {
       const U t = reinterpret_cast<const U *>(p)[i];
 #pragma unroll
       for (int k = 0; k < W; ++k) v[W * i + k] = t.v[k];
+    }
+  } else if constexpr (sizeof(T) == 4 && N % 2 == 0) {
+    struct alignas(8) U { T v[2]; };
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) {
+      const U t = reinterpret_cast<const U *>(p)[i];
+      v[2 * i] = t.v[0];
+      v[2 * i + 1] = t.v[1];
     }
   } else {
 #pragma unroll
