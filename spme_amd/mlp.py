@@ -1,15 +1,37 @@
-"""Learned parameter selection: a 4 -> 64 -> 64 -> 4 MLP trained and run by
+"""Learned parameter selection: 4 -> 64 -> 64 -> 4 MLPs trained and run by
 the engine's own HIP kernels (spme_amd/csrc/spme_mlp.hip).
 
-Inputs  (log N, phi, log10 tol, L/a), standardised.
-Outputs (log xi, log2 K, p, log rc), standardised; K is snapped to the
-        smooth set {32, 48, 64, 96, 128, 192, 256}, p to {4, 6, 8}.
+The tolerance enters as an absolute tolerance, tol_abs = tol * rms(u)/rms(f)
+with rms(u)/rms(f) = sqrt(1 + 0.66 phi L) (spme_amd/params.py): the relative
+error of a fixed parameter set falls as the box grows (the long-range 1/r
+field grows with it), while the absolute error per unit force is intensive.
 
-Labels come from measured SPME runs (tests/golden/make_mlp_data.py): for
-each system and tolerance, the fastest evaluated parameter set whose
-measured relative error against the FP64 direct Ewald sum is within the
-tolerance.  The analytic estimate (spme_amd/params.py) is the fallback when
-no trained weights are present or when the prediction is invalid.
+Full selector (mesh, order, xi and rc free):
+  inputs  (log N, phi, log10 tol_abs, L/a), standardised;
+  outputs (log xi, log h, p, log rc), standardised, with h = L/K the mesh
+          spacing: K is the smooth size in {32, 48, 64, 96, 128, 192, 256}
+          nearest to L/h in log (the labels are such sizes), p is snapped to
+          {4, 6, 8}.
+  Labels come from measured SPME runs (tests/golden/make_mlp_data.py): for
+  each system and tolerance, the fastest evaluated parameter set whose
+  measured relative error against the FP64 direct Ewald sum is within the
+  tolerance.  At fixed (xi, h, p, rc) SPME's absolute error and its
+  per-particle cost are intensive, so the outputs carry over to larger
+  systems: inputs outside the training box are clamped to its edge (N
+  clamped and L recomputed from it and phi, so the inputs stay a physical
+  system; tol_abs from the true box, then clamped), and when the mesh the
+  spacing asks for exceeds the largest one allowed, xi h and xi rc are kept.
+
+Fixed-mesh selector (K and p given, xi and rc free):
+  inputs  (phi, log10 tol_abs, h, p), standardised; outputs (log xi, log rc)
+          (the last two outputs are unused and masked out of the loss).
+  Labels come from runs over a grid of xi and rc at fixed meshes
+  (tests/golden/make_mesh_runs.py): the xi whose measured error falls within
+  the tolerance at the smallest rc (at a fixed mesh the cost grows with rc
+  only).
+
+The analytic estimate (spme_amd/params.py) is the fallback when no trained
+weights are present or when a prediction is unusable.
 """
 from __future__ import annotations
 
@@ -25,16 +47,46 @@ from .lib import check, load
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "data", "mlp_weights.npz")
+MESH_WEIGHTS = os.path.join(HERE, "data", "mlp_mesh_weights.npz")
 NI, NH, NO = 4, 64, 4
 
 
-def features(n, phi, tol):
-    L = PR.box_length(n, phi)
-    return np.array([math.log(n), phi, math.log10(tol), L], dtype=np.float64)
+def tol_abs(tol, phi, L):
+    """Absolute tolerance per unit rms force of a relative tolerance."""
+    return tol * PR.rms_u(phi, L)
 
 
-def targets(xi, K, p, rc):
-    return np.array([math.log(xi), math.log2(K), float(p), math.log(rc)], dtype=np.float64)
+def features(n, phi, tol, L=None):
+    """(log N, phi, log10 tol_abs, L); tol_abs from the box L (default: the
+    box of N and phi)."""
+    Lb = PR.box_length(n, phi)
+    return np.array([math.log(n), phi, math.log10(tol_abs(tol, phi, Lb if L is None else L)),
+                     Lb], dtype=np.float64)
+
+
+def targets(xi, K, p, rc, L):
+    return np.array([math.log(xi), math.log(L / K), float(p), math.log(rc)], dtype=np.float64)
+
+
+def mesh_features(phi, tol, h, p, L):
+    return np.array([phi, math.log10(tol_abs(tol, phi, L)), h, float(p)], dtype=np.float64)
+
+
+def mesh_targets(xi, rc):
+    return np.array([math.log(xi), math.log(rc), 0.0, 0.0], dtype=np.float64)
+
+
+MESH_MASK = (1.0, 1.0, 0.0, 0.0)
+
+
+def _clamp(v, lo, hi):
+    return min(max(v, lo), hi)
+
+
+def mesh_for(L, h, k_max=256):
+    """The smooth K <= k_max nearest to L/h in log."""
+    ks = [k for k in PR.SMOOTH_K if k <= k_max]
+    return min(ks, key=lambda k: abs(math.log(k * h / L)))
 
 
 def nparam() -> int:
@@ -114,36 +166,60 @@ class Net:
         return hist
 
     # -- selection -------------------------------------------------------
-    def in_domain(self, n, phi, tol):
-        """True inside the (N, phi, tol) box the network was trained on; the
-        network is not trusted to extrapolate outside it."""
-        d = getattr(self, "meta", {}).get("domain")
-        if not d:
-            return True
-        return (d["n"][0] <= n <= d["n"][1] and d["phi"][0] <= phi <= d["phi"][1]
-                and d["tol"][0] <= tol <= d["tol"][1])
+    def _domain(self):
+        return getattr(self, "meta", {}).get("domain") or {}
+
+    def predict_x(self, x):
+        xs = (np.asarray(x, np.float64) - self.x_mean) / self.x_std
+        return self.forward_raw(xs[None, :])[0] * self.y_std + self.y_mean
 
     def predict(self, n, phi, tol):
-        xs = (features(n, phi, tol) - self.x_mean) / self.x_std
-        ys = self.forward_raw(xs[None, :])[0] * self.y_std + self.y_mean
-        return ys
+        """Raw outputs (log xi, log h, p, log rc) for a system, its inputs
+        clamped to the training box (N clamped and L recomputed from it;
+        tol_abs from the true box, then clamped)."""
+        x = features(n, phi, tol)
+        d = self._domain()
+        if d:
+            nc, phic = _clamp(n, *d["n"]), _clamp(phi, *d["phi"])
+            x = features(nc, phic, 1.0)
+            x[2] = _clamp(math.log10(tol_abs(tol, phi, PR.box_length(n, phi))),
+                          *d["log10_tol_abs"])
+        return self.predict_x(x)
 
     def select(self, n, phi, tol, nrhs=1, k_max=256):
-        """(xi, K, p, rc) for the target relative error tol; falls back to the
-        analytic estimate if the prediction is unusable."""
+        """(xi, K, p, rc, how) for the target relative error tol (full
+        selector); how is "learned", or "analytic" for the fallback."""
         L = PR.box_length(n, phi)
-        lxi, lk, pp, lrc = self.predict(n, phi, tol)
-        xi, rc = math.exp(lxi), math.exp(lrc)
-        Kc = [k for k in PR.SMOOTH_K if k <= k_max]
-        K = min(Kc, key=lambda k: abs(math.log2(k) - lk))
+        lxi, lh, pp, lrc = self.predict(n, phi, tol)
+        xi, h, rc = math.exp(lxi), math.exp(lh), math.exp(lrc)
+        K = mesh_for(L, h, k_max)
+        if K == max(k for k in PR.SMOOTH_K if k <= k_max) and L / K > h:
+            # the mesh is capped: keep xi h (mesh error) and xi rc (real space)
+            s = h / (L / K)
+            xi, rc = xi * s, rc / s
         p = min(PR.ORDERS, key=lambda q: abs(q - pp))
         while 4 * p >= K and p > 4:
             p -= 2
         ok = (0.05 < xi < 3.0) and (0.5 < rc <= L / 3.0) and 4 * p < K
-        if not ok or not self.in_domain(n, phi, tol):
+        if not ok:
             a = PR.analytic_select(n, phi, tol, nrhs=nrhs, k_max=k_max)
             return a.xi, a.K, a.p, a.rc, "analytic"
         return xi, K, p, rc, "learned"
+
+    def select_mesh(self, n, phi, tol, K, p):
+        """(xi, rc, how) at a given mesh and order (fixed-mesh selector)."""
+        L = PR.box_length(n, phi)
+        x = mesh_features(phi, tol, L / K, p, L)
+        d = self._domain()
+        if d:
+            x = np.array([_clamp(x[0], *d["phi"]), _clamp(x[1], *d["log10_tol_abs"]),
+                          _clamp(x[2], *d["h"]), x[3]])
+        lxi, lrc = self.predict_x(x)[:2]
+        xi, rc = math.exp(lxi), math.exp(lrc)
+        if not ((0.05 < xi < 3.0) and (0.5 < rc <= L / 3.0)):
+            a = PR.analytic_select(n, phi, tol, fixed_K=K, fixed_p=p)
+            return a.xi, a.rc, "analytic"
+        return xi, rc, "learned"
 
     # -- storage -----------------------------------------------------------
     def save(self, path=WEIGHTS, meta=None):
@@ -168,42 +244,104 @@ TOLS = (1e-3, 3e-4, 1e-4, 3e-5, 1e-5, 3e-6, 1e-6, 3e-7, 1e-7)
 
 def labels_from_runs(runs, tols=TOLS, margin=1.0):
     """For every system and tolerance: the fastest run whose measured error is
-    within tol / margin.  runs: dict of equal-length arrays with keys
-    sys, n, phi, xi, K, p, rc, err, time.  Returns (X, T, info)."""
+    within tol / margin.  runs: dict of equal-length arrays with keys sys, n,
+    phi, xi, K, p, rc, err, time.  Returns (X, T, info)."""
     X, T, info = [], [], []
-    sysids = np.unique(runs["sys"])
-    for s in sysids:
+    for s in np.unique(runs["sys"]):
         sel = np.where(runs["sys"] == s)[0]
         for tol in tols:
             ok = sel[runs["err"][sel] <= tol / margin]
             if ok.size == 0:
                 continue
             b = ok[np.argmin(runs["time"][ok])]
-            X.append(features(int(runs["n"][b]), float(runs["phi"][b]), tol))
-            T.append(targets(runs["xi"][b], runs["K"][b], runs["p"][b], runs["rc"][b]))
+            n, phi = int(runs["n"][b]), float(runs["phi"][b])
+            X.append(features(n, phi, tol))
+            T.append(targets(runs["xi"][b], runs["K"][b], runs["p"][b], runs["rc"][b],
+                             PR.box_length(n, phi)))
             info.append((int(s), tol, int(b)))
     return np.array(X), np.array(T), info
 
 
-def training_domain(runs, tols=TOLS):
-    """The (N, phi, tol) box covered by the training runs."""
+def training_domain(X, runs):
+    """The box covered by the training runs: N, phi and log10 tol_abs."""
     return {"n": [int(runs["n"].min()), int(runs["n"].max())],
             "phi": [float(runs["phi"].min()), float(runs["phi"].max())],
-            "tol": [float(min(tols)), float(max(tols))]}
+            "log10_tol_abs": [float(X[:, 2].min()), float(X[:, 2].max())]}
 
 
-def fit(runs, epochs=4000, lr=3e-3, seed=0, log_every=500, margin=1.5):
-    """Train a network on measured runs; returns (net, history).  Labels keep
-    a margin below the tolerance so that predictions, which interpolate
-    between neighbouring systems, still land inside it."""
-    X, T, _ = labels_from_runs(runs, margin=margin)
+def mesh_labels(runs, tols=TOLS, margin=1.5):
+    """Fixed-mesh labels.  For every (system, K, p), tolerance and xi of the
+    grid: the smallest rc from which on the measured error stays within
+    tol / margin, log-interpolated between the two grid cutoffs around the
+    crossing; the label is the xi with the smallest such rc.  runs: keys
+    sys, n, phi, L, K, p, xi, rc, err.  Returns (X, T, info)."""
+    X, T, info = [], [], []
+    key = np.stack([runs["sys"], runs["K"], runs["p"]], 1)
+    for k in np.unique(key, axis=0):
+        sel = np.where((key == k).all(1))[0]
+        s, K, p = (int(v) for v in k)
+        L, phi = float(runs["L"][sel[0]]), float(runs["phi"][sel[0]])
+        xis = np.unique(runs["xi"][sel])
+        for tol in tols:
+            target = tol / margin
+            best = None
+            for xi in xis:
+                q = sel[runs["xi"][sel] == xi]
+                q = q[np.argsort(runs["rc"][q])]
+                rc, err = runs["rc"][q], runs["err"][q]
+                above = np.where(err > target)[0]
+                i = 0 if above.size == 0 else above[-1] + 1
+                if i >= rc.size:
+                    continue
+                r = rc[i]
+                if i > 0:  # crossing between rc[i-1] (above) and rc[i]
+                    la, lb = math.log(err[i - 1]), math.log(max(err[i], 1e-300))
+                    t = (la - math.log(target)) / (la - lb) if la > lb else 1.0
+                    r = rc[i - 1] + t * (rc[i] - rc[i - 1])
+                if best is None or r < best[1]:
+                    best = (float(xi), float(r))
+            if best is None:
+                continue
+            X.append(mesh_features(phi, tol, L / K, p, L))
+            T.append(mesh_targets(*best))
+            info.append((s, K, p, tol))
+    return np.array(X), np.array(T), info
+
+
+def _train(X, T, mask, epochs, lr, seed, log_every):
     xm, xsd = X.mean(0), X.std(0) + 1e-12
     ym, ysd = T.mean(0), T.std(0) + 1e-12
     net = Net(init_params(seed), xm, xsd, ym, ysd)
-    hist = net.train((X - xm) / xsd, (T - ym) / ysd, epochs=epochs, lr=lr,
+    hist = net.train((X - xm) / xsd, (T - ym) / ysd, mask=mask, epochs=epochs, lr=lr,
                      log_every=log_every)
-    net.meta = {"samples": int(len(X)), "epochs": epochs, "lr": lr, "seed": seed,
-                "margin": margin, "domain": training_domain(runs),
+    return net, hist
+
+
+def fit(runs, epochs=4000, lr=3e-3, seed=0, log_every=500, margin=1.5):
+    """Train the full selector on measured runs; returns (net, history).
+    Labels keep a margin below the tolerance so that predictions, which
+    interpolate between neighbouring systems, still land inside it."""
+    X, T, _ = labels_from_runs(runs, margin=margin)
+    net, hist = _train(X, T, (1, 1, 1, 1), epochs, lr, seed, log_every)
+    net.meta = {"kind": "full", "samples": int(len(X)), "epochs": epochs, "lr": lr,
+                "seed": seed, "margin": margin, "domain": training_domain(X, runs),
+                "final_loss": hist[-1][1] if hist else None}
+    return net, hist
+
+
+def fit_mesh(runs, epochs=4000, lr=3e-3, seed=0, log_every=500, margin=1.25):
+    """Train the fixed-mesh selector on grid runs; returns (net, history).
+    Its labels are read off dense (xi, rc) grids, at the tolerance itself
+    rather than at the best of a few candidates, so a smaller margin than
+    the full selector's holds them inside it."""
+    X, T, _ = mesh_labels(runs, margin=margin)
+    net, hist = _train(X, T, MESH_MASK, epochs, lr, seed, log_every)
+    h = runs["L"] / runs["K"]
+    net.meta = {"kind": "mesh", "samples": int(len(X)), "epochs": epochs, "lr": lr,
+                "seed": seed, "margin": margin,
+                "domain": {"phi": [float(runs["phi"].min()), float(runs["phi"].max())],
+                           "log10_tol_abs": [float(X[:, 1].min()), float(X[:, 1].max())],
+                           "h": [float(h.min()), float(h.max())]},
                 "final_loss": hist[-1][1] if hist else None}
     return net, hist
 
@@ -216,6 +354,15 @@ def select(n, phi, tol, nrhs=1, k_max=256):
     return a.xi, a.K, a.p, a.rc, "analytic"
 
 
+def select_mesh(n, phi, tol, K, p):
+    """(xi, rc, how) at a fixed mesh and order: the fixed-mesh network if its
+    weights are present, else the analytic estimate."""
+    if os.path.exists(MESH_WEIGHTS):
+        return Net.load(MESH_WEIGHTS).select_mesh(n, phi, tol, K, p)
+    a = PR.analytic_select(n, phi, tol, fixed_K=K, fixed_p=p)
+    return a.xi, a.rc, "analytic"
+
+
 def load_runs(*paths):
     """Concatenate run files written by the training-data generator."""
     parts = [np.load(p) for p in paths]
@@ -223,12 +370,13 @@ def load_runs(*paths):
 
 
 if __name__ == "__main__":
-    # python -m spme_amd.mlp OUT.npz RUNS.npz [RUNS.npz ...]
+    # python -m spme_amd.mlp full|mesh OUT.npz RUNS.npz [RUNS.npz ...]
     import sys
-    runs = load_runs(*sys.argv[2:])
-    net, hist = fit(runs)
+    kind, out = sys.argv[1], sys.argv[2]
+    runs = load_runs(*sys.argv[3:])
+    net, hist = (fit if kind == "full" else fit_mesh)(runs)
     for ep, loss in hist:
         print(f"epoch {ep} loss {loss:.4e}", flush=True)
     net.meta["runs"] = int(len(runs["err"]))
-    net.save(sys.argv[1], meta=net.meta)
-    print("saved", sys.argv[1], net.meta, flush=True)
+    net.save(out, meta=net.meta)
+    print("saved", out, net.meta, flush=True)

@@ -58,7 +58,32 @@ def test_labels_pick_fastest_feasible():
             "time": np.array([2.0, 1.0, 3.0])}
     X, T, info = mlp.labels_from_runs(runs, tols=(1e-3, 1e-5, 1e-6))
     assert [i[2] for i in info] == [1, 0]          # tol 1e-6 infeasible
-    assert np.allclose(T[0], mlp.targets(0.4, 48, 6, 7.0))
+    L = PR.box_length(1000, 0.1)
+    assert np.allclose(T[0], mlp.targets(0.4, 48, 6, 7.0, L))
+    assert np.isclose(np.exp(T[0][1]), L / 48)     # the mesh spacing, not K
+    # the tolerance enters in absolute terms: tol * sqrt(1 + 0.66 phi L)
+    assert np.isclose(10 ** X[0][2], 1e-3 * np.sqrt(1 + 0.66 * 0.1 * L))
+
+
+def test_mesh_labels_interpolate_the_crossing():
+    """The smallest cutoff at which the error stays within the target,
+    log-interpolated between grid cutoffs; the xi reaching it first wins."""
+    from spme_amd import mlp
+    rc = np.array([2.0, 4.0, 6.0, 8.0])
+    e1 = np.array([1e-1, 1e-3, 1e-5, 1e-5])   # xi = 0.5: crosses 1e-4 between 4 and 6
+    e2 = np.array([1e-2, 1e-4 * 1.5, 2e-4, 2e-4])  # xi = 0.9: floor above the target
+    runs = {k: np.concatenate([v, v]) for k, v in
+            dict(sys=np.zeros(4, int), K=np.full(4, 64), p=np.full(4, 6),
+                 L=np.full(4, 50.0), phi=np.full(4, 0.2), rc=rc).items()}
+    runs["xi"] = np.array([0.5] * 4 + [0.9] * 4)
+    runs["err"] = np.concatenate([e1, e2])
+    X, T, info = mlp.mesh_labels(runs, tols=(1.5e-4,), margin=1.5)
+    assert len(X) == 1
+    assert np.isclose(np.exp(T[0][0]), 0.5)
+    assert np.isclose(np.exp(T[0][1]), 5.0)          # halfway in log error from 1e-3 to 1e-5
+    assert np.allclose(X[0], mlp.mesh_features(0.2, 1.5e-4, 50.0 / 64, 6, 50.0))
+    assert mlp.mesh_for(191.1, 0.6) == 256 and mlp.mesh_for(100.0, 0.8) == 128
+    assert mlp.mesh_for(100.0, 0.95) == 96 and mlp.mesh_for(100.0, 0.5, k_max=128) == 128
 
 
 @pytest.mark.gpu
