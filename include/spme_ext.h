@@ -50,8 +50,12 @@ typedef struct spme_params {
   int np;          /* number of particles N, at most 2^23 */
   double box_size; /* cube side L */
   double rc;       /* real-space cutoff (used by SPME_REAL) */
-  int precision;   /* SPME_FP64, SPME_FP32 or SPME_FP64_PLAIN (see above) */
+  int precision;   /* SPME_FP64, SPME_FP32, SPME_FP64_PLAIN or SPME_FP32_SUM64 */
   int max_rhs;     /* largest batch of right-hand sides per apply */
+  int radii;       /* nonzero: per-particle radii (spme_engine_set_radii, 1 until
+                      set); the mesh then carries f and a^2 f, six components per
+                      right-hand side, so max_rhs <= 8; not for the plain path,
+                      groups or debug stages */
 } spme_params_t;
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -64,6 +68,15 @@ int spme_engine_destroy(spme_t *spme);
  * device pointer if on_device != 0, else host memory.  stream is a
  * hipStream_t (NULL: the default stream); the call waits for the build. */
 int spme_engine_set_positions(spme_t *spme, const double *pos, int on_device, void *stream);
+
+/* Per-particle sphere radii (N FP64 values, in units of the reference
+ * radius 1; device pointer if on_device) for an engine created with
+ * radii != 0.  The reference's psi(k) takes a pair's radii aa, ab
+ * (old-version/spme.c:43-60) but is only called with 1, 1 (spme.c:141);
+ * here psi_ij uses (a_i^2 + a_j^2)/2, the real-space part its Ewald split,
+ * the overlap correction the unequal-sphere RPY tensor of Zuk et al. (2014)
+ * and the self term 1/a_i.  Positions must be set (again) after radii. */
+int spme_engine_set_radii(spme_t *spme, const double *radii, int on_device, void *stream);
 
 /* out[r*ldout + 3i + c] <- beta*out + alpha*(M f_r)[3i + c], r < nrhs,
  * for the parts in `parts`.  in/out are device pointers of the engine's

@@ -61,6 +61,15 @@ def lib():
         L.orc_ewald_direct.argtypes = [i, _dp, d, d, d, d, _dp, i, _ip, _dp]
         L.orc_spme_full.argtypes = [d, i, i, d, i, d, _dp, i, d, _dp, i, d,
                                     _dp, i]
+        L.orc_real_coeffs_split.argtypes = [d, d, _dp, _dp, _dp, _dp]
+        L.orc_overlap_coeffs_radii.argtypes = [d, d, d, _dp, _dp]
+        L.orc_self_coeff_radius.argtypes = [d, d]
+        L.orc_self_coeff_radius.restype = d
+        L.orc_real_space_radii.argtypes = [i, _dp, _dp, d, d, d, _dp, i, _ip, _dp]
+        L.orc_recip_exact_radii.argtypes = [i, _dp, _dp, d, d, d, _dp, i, _ip, _dp]
+        L.orc_ewald_direct_radii.argtypes = [i, _dp, _dp, d, d, d, d, _dp, i, _ip, _dp]
+        L.orc_spme_full_radii.argtypes = [d, i, i, d, i, d, _dp, _dp, i, d, _dp, i, d,
+                                          _dp, i]
         L.orc_num_threads.restype = i
         _lib = L
     return _lib
@@ -306,6 +315,77 @@ def spme_full(xi, K, p, rc, L, pos, f, alpha=1.0, beta=0.0, u=None):
     lib().orc_spme_full(float(xi), K, p, float(rc), N, float(L), _d(pos), nrhs,
                         float(alpha), _d(f), 3 * N, float(beta), _d(out),
                         3 * N)
+    return out
+
+
+# --- unequal radii (spme_oracle.c: orc_*_radii) ------------------------------
+
+def real_coeffs_split(r, xi):
+    """(F_J, G_J, F_s, G_s): the Oseen and s2 = (a_i^2 + a_j^2)/2 parts of the
+    real-space tensor, F = F_J + s2 F_s, G = G_J + s2 G_s."""
+    v = [ctypes.c_double() for _ in range(4)]
+    lib().orc_real_coeffs_split(float(r), float(xi), *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
+
+
+def overlap_coeffs_radii(r, ai, aj):
+    F, G = ctypes.c_double(), ctypes.c_double()
+    lib().orc_overlap_coeffs_radii(float(r), float(ai), float(aj), ctypes.byref(F),
+                                   ctypes.byref(G))
+    return F.value, G.value
+
+
+def self_coeff_radius(xi, a):
+    return lib().orc_self_coeff_radius(float(xi), float(a))
+
+
+def real_space_radii(pos, rad, L, xi, rc, f, rows=None):
+    pos = _c64(pos).reshape(-1)
+    N = pos.size // 3
+    rad, f = _c64(rad).reshape(-1), _c64(f).reshape(-1)
+    r, n = _rows(rows, N)
+    u = np.zeros(3 * n)
+    lib().orc_real_space_radii(N, _d(pos), _d(rad), float(L), float(xi), float(rc), _d(f), n,
+                               None if r is None else _i(r), _d(u))
+    return u
+
+
+def recip_exact_radii(pos, rad, L, xi, kmax, f, rows=None):
+    pos = _c64(pos).reshape(-1)
+    N = pos.size // 3
+    rad, f = _c64(rad).reshape(-1), _c64(f).reshape(-1)
+    r, n = _rows(rows, N)
+    u = np.zeros(3 * n)
+    lib().orc_recip_exact_radii(N, _d(pos), _d(rad), float(L), float(xi), float(kmax), _d(f),
+                                n, None if r is None else _i(r), _d(u))
+    return u
+
+
+def ewald_direct_radii(pos, rad, L, f, rows=None, xi=None, tol_exp=6.6):
+    """FP64 direct Ewald M.f with per-particle radii (see ewald_direct)."""
+    pos = _c64(pos).reshape(-1)
+    N = pos.size // 3
+    rad, f = _c64(rad).reshape(-1), _c64(f).reshape(-1)
+    r, n = _rows(rows, N)
+    if xi is None:
+        xi = default_direct_xi(N, L, n)
+    u = np.zeros(3 * n)
+    lib().orc_ewald_direct_radii(N, _d(pos), _d(rad), float(L), float(xi), tol_exp / xi,
+                                 2.0 * xi * tol_exp, _d(f), n, None if r is None else _i(r),
+                                 _d(u))
+    return u
+
+
+def spme_full_radii(xi, K, p, rc, L, pos, rad, f, alpha=1.0, beta=0.0, u=None):
+    """Full SPME M.f with per-particle radii on the CPU."""
+    pos = _c64(pos).reshape(-1)
+    N = pos.size // 3
+    rad = _c64(rad).reshape(-1)
+    f = _c64(f).reshape(-1, 3 * N)
+    nrhs = f.shape[0]
+    out = np.zeros_like(f) if u is None else _c64(u).reshape(nrhs, 3 * N).copy()
+    lib().orc_spme_full_radii(float(xi), K, p, float(rc), N, float(L), _d(pos), _d(rad), nrhs,
+                              float(alpha), _d(f), 3 * N, float(beta), _d(out), 3 * N)
     return out
 
 

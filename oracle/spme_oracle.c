@@ -765,6 +765,294 @@ void orc_spme_full(double xi, int K, int p, double rc, int N, double L,
     free(t);
 }
 
+/* ------------------------------------------------------------------------ */
+/* Unequal radii (not built by the reference: its psi takes the pair's radii
+ * aa, ab, spme.c:43-60, but is only called with 1, 1, spme.c:141).  Units:
+ * 1/(6 pi eta) with lengths in units of the reference radius 1.
+ *
+ * The RPY far field of spheres a_i, a_j is linear in s2 = (a_i^2 + a_j^2)/2:
+ *   M_far = 3/(4r) (I + rr^T/r^2) + s2/(2r^3) (I - 3 rr^T/r^2),
+ * so every Ewald part splits into an Oseen part and an s2 part.  Real space:
+ * Beenakker's terms (orc_real_coeffs, s2 = 1) split by dimension,
+ *   F = F_J + s2 F_s,  G = G_J + s2 G_s,
+ *   F_J = erfc(xi r) 3/(4r) + e (3 xi^3 r^2 - 9 xi/2)
+ *   F_s = erfc(xi r)/(2r^3) + e (4 xi^7 r^4 - 20 xi^5 r^2 + 14 xi^3 + xi/r^2)
+ *   G_J = erfc(xi r) 3/(4r) + e (-3 xi^3 r^2 + 3 xi/2)
+ *   G_s = -erfc(xi r) 3/(2r^3) + e (-4 xi^7 r^4 + 16 xi^5 r^2 - 2 xi^3 - 3 xi/r^2)
+ * with e = exp(-xi^2 r^2)/sqrt(pi).  Wave space: psi_ij(k) = A(k)(1 - k^2
+ * (a_i^2 + a_j^2)/6) with A = scalar_recip(k, xi, 0, 0), so
+ *   u_i = I[A P (S_f - k^2/6 S_{a^2 f})](r_i) - a_i^2 I[(k^2/6) A P S_f](r_i)
+ * (S the structure factor, P = I - kk^T/k^2).  Self: 1/a_i - 6 xi/sqrt(pi)
+ * + 40 xi^3 a_i^2/(3 sqrt(pi)).  Overlap (Zuk, Wajnryb, Mizerski and
+ * Szymczak, J. Fluid Mech. 741 (2014) R5), r < a_i + a_j: M_ov - M_far with
+ *   |a_i - a_j| < r:  M_ov = [16 r^3 (a_i + a_j) - ((a_i - a_j)^2 + 3r^2)^2]
+ *                           / (32 r^3 a_i a_j) I
+ *                          + 3((a_i - a_j)^2 - r^2)^2 / (32 r^3 a_i a_j) rr^T/r^2
+ *   r <= |a_i - a_j|: M_ov = I / max(a_i, a_j).                             */
+
+void orc_real_coeffs_split(double r, double xi, double *FJ, double *GJ, double *Fs,
+                           double *Gs)
+{
+    double r2 = r * r, xr = xi * r, e = exp(-xr * xr) / sqrt(M_PI);
+    double ec = erfc(xr);
+    double x3 = xi * xi * xi, x5 = x3 * xi * xi, x7 = x5 * xi * xi;
+    *FJ = ec * (0.75 / r) + e * (3.0 * x3 * r2 - 4.5 * xi);
+    *Fs = ec * (0.5 / (r2 * r)) + e * (4.0 * x7 * r2 * r2 - 20.0 * x5 * r2 + 14.0 * x3 + xi / r2);
+    *GJ = ec * (0.75 / r) + e * (-3.0 * x3 * r2 + 1.5 * xi);
+    *Gs = -ec * (1.5 / (r2 * r)) + e * (-4.0 * x7 * r2 * r2 + 16.0 * x5 * r2 - 2.0 * x3
+                                        - 3.0 * xi / r2);
+}
+
+void orc_overlap_coeffs_radii(double r, double ai, double aj, double *F, double *G)
+{
+    double s2 = 0.5 * (ai * ai + aj * aj), d = ai - aj, r3 = r * r * r;
+    double Fov, Gov;
+    if (r >= ai + aj) { *F = 0.0; *G = 0.0; return; }
+    if (r <= fabs(d)) {
+        Fov = 1.0 / (ai > aj ? ai : aj);
+        Gov = 0.0;
+    } else {
+        double q = d * d + 3.0 * r * r, w = d * d - r * r;
+        Fov = (16.0 * r3 * (ai + aj) - q * q) / (32.0 * r3 * ai * aj);
+        Gov = 3.0 * w * w / (32.0 * r3 * ai * aj);
+    }
+    *F = Fov - (0.75 / r + s2 / (2.0 * r3));
+    *G = Gov - (0.75 / r - 1.5 * s2 / r3);
+}
+
+double orc_self_coeff_radius(double xi, double a)
+{
+    return 1.0 / a - 6.0 * xi / sqrt(M_PI) + 40.0 * xi * xi * xi * a * a / (3.0 * sqrt(M_PI));
+}
+
+/* Real-space sum with radii for rows[0..nrows), added into u: pairs within
+ * rc (every periodic image) and the overlap correction on the minimum image
+ * (r < a_i + a_j).  All pairs are visited (brute force, O(nrows N)).       */
+void orc_real_space_radii(int N, const double *pos, const double *rad, double L, double xi,
+                          double rc, const double *f, int nrows, const int *rows, double *u)
+{
+    int t;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (t = 0; t < nrows; t++) {
+        int i = rows ? rows[t] : t, j, m;
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (j = 0; j < N; j++) {
+            double d0[3], s2 = 0.5 * (rad[i] * rad[i] + rad[j] * rad[j]);
+            int lo[3], hi[3], a, b, c;
+            min_image(pos + 3 * i, pos + 3 * j, L, d0);
+            for (m = 0; m < 3; m++) {
+                lo[m] = (int)ceil((-rc - d0[m]) / L);
+                hi[m] = (int)floor((rc - d0[m]) / L);
+            }
+            for (a = lo[0]; a <= hi[0]; a++)
+            for (b = lo[1]; b <= hi[1]; b++)
+            for (c = lo[2]; c <= hi[2]; c++) {
+                double d[3] = {d0[0] + a * L, d0[1] + b * L, d0[2] + c * L};
+                double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2], FJ, GJ, Fs, Gs;
+                if (r2 == 0.0 || r2 >= rc * rc) continue;
+                orc_real_coeffs_split(sqrt(r2), xi, &FJ, &GJ, &Fs, &Gs);
+                add_block(FJ + s2 * Fs, GJ + s2 * Gs, d, r2, f + 3 * j, acc);
+            }
+            if (j != i) {
+                double r2 = d0[0] * d0[0] + d0[1] * d0[1] + d0[2] * d0[2], F, G;
+                double sa = rad[i] + rad[j];
+                if (r2 > 0.0 && r2 < sa * sa) {
+                    orc_overlap_coeffs_radii(sqrt(r2), rad[i], rad[j], &F, &G);
+                    add_block(F, G, d0, r2, f + 3 * j, acc);
+                }
+            }
+        }
+        u[3 * t + 0] += acc[0];
+        u[3 * t + 1] += acc[1];
+        u[3 * t + 2] += acc[2];
+    }
+}
+
+/* Exact wave-space sum with radii (see orc_recip_exact): the structure
+ * factors of f and of a^2 f, u_i += (1/V) sum_k A(k) P Re[e^{ik.r_i}
+ * (S_f - k^2/6 (a_i^2 S_f + S_{a^2 f}))], over the half space, doubled.    */
+void orc_recip_exact_radii(int N, const double *pos, const double *rad, double L, double xi,
+                           double kmax, const double *f, int nrows, const int *rows,
+                           double *u)
+{
+    double dk = 2.0 * M_PI / L, V = L * L * L;
+    int nmax = (int)floor(kmax / dk), nz;
+    double *ur = (double *)calloc((size_t)3 * nrows, sizeof(double));
+#pragma omp parallel
+    {
+        double *loc = (double *)calloc((size_t)3 * nrows, sizeof(double));
+#pragma omp for schedule(dynamic, 1)
+        for (nz = 0; nz <= nmax; nz++) {
+            int ny, nx, j, t, m;
+            for (ny = -nmax; ny <= nmax; ny++)
+            for (nx = -nmax; nx <= nmax; nx++) {
+                double kx = nx * dk, ky = ny * dk, kz = nz * dk;
+                double k2 = kx * kx + ky * ky + kz * kz, kv[3] = {kx, ky, kz}, A;
+                double S[6] = {0, 0, 0, 0, 0, 0}, Sa[6] = {0, 0, 0, 0, 0, 0};
+                if (k2 >= kmax * kmax || k2 == 0.0) continue;
+                if (nz == 0 && (ny < 0 || (ny == 0 && nx < 0))) continue;
+                for (j = 0; j < N; j++) {
+                    double th = kx * pos[3 * j] + ky * pos[3 * j + 1] + kz * pos[3 * j + 2];
+                    double c = cos(th), sn = -sin(th), a2 = rad[j] * rad[j];
+                    for (m = 0; m < 3; m++) {
+                        S[2 * m] += c * f[3 * j + m];
+                        S[2 * m + 1] += sn * f[3 * j + m];
+                        Sa[2 * m] += a2 * c * f[3 * j + m];
+                        Sa[2 * m + 1] += a2 * sn * f[3 * j + m];
+                    }
+                }
+                /* project both onto the plane normal to k */
+                {
+                    double sr = (S[0] * kx + S[2] * ky + S[4] * kz) / k2;
+                    double si = (S[1] * kx + S[3] * ky + S[5] * kz) / k2;
+                    double ar = (Sa[0] * kx + Sa[2] * ky + Sa[4] * kz) / k2;
+                    double ai = (Sa[1] * kx + Sa[3] * ky + Sa[5] * kz) / k2;
+                    for (m = 0; m < 3; m++) {
+                        S[2 * m] -= sr * kv[m];
+                        S[2 * m + 1] -= si * kv[m];
+                        Sa[2 * m] -= ar * kv[m];
+                        Sa[2 * m + 1] -= ai * kv[m];
+                    }
+                }
+                A = 2.0 * orc_scalar_recip(sqrt(k2), xi, 0.0, 0.0) / V;
+                for (t = 0; t < nrows; t++) {
+                    int i = rows ? rows[t] : t;
+                    double th = kx * pos[3 * i] + ky * pos[3 * i + 1] + kz * pos[3 * i + 2];
+                    double c = cos(th), sn = sin(th), w = 1.0 - k2 * rad[i] * rad[i] / 6.0;
+                    for (m = 0; m < 3; m++) {
+                        double gr = w * S[2 * m] - k2 / 6.0 * Sa[2 * m];
+                        double gi = w * S[2 * m + 1] - k2 / 6.0 * Sa[2 * m + 1];
+                        loc[3 * t + m] += A * (c * gr - sn * gi);
+                    }
+                }
+            }
+        }
+#pragma omp critical
+        {
+            int t;
+            for (t = 0; t < 3 * nrows; t++) ur[t] += loc[t];
+        }
+        free(loc);
+    }
+    {
+        int t;
+        for (t = 0; t < 3 * nrows; t++) u[t] += ur[t];
+    }
+    free(ur);
+}
+
+/* FP64 direct Ewald mobility product with radii (overwrites u). */
+void orc_ewald_direct_radii(int N, const double *pos, const double *rad, double L, double xi,
+                            double rc, double kmax, const double *f, int nrows,
+                            const int *rows, double *u)
+{
+    int t, m;
+    for (t = 0; t < nrows; t++) {
+        int i = rows ? rows[t] : t;
+        double s = orc_self_coeff_radius(xi, rad[i]);
+        for (m = 0; m < 3; m++) u[3 * t + m] = s * f[3 * i + m];
+    }
+    orc_real_space_radii(N, pos, rad, L, xi, rc, f, nrows, rows, u);
+    orc_recip_exact_radii(N, pos, rad, L, xi, kmax, f, nrows, rows, u);
+}
+
+/* Full SPME mobility product with radii: the reference's mesh steps (P,
+ * spread, FFT, influence, inverse FFT, interpolation) applied to f and to
+ * a^2 f, the influence of the pair (A |b|^2 from scalar_recip with zero
+ * radii), u = beta u + alpha (I[G1] - a_i^2 I[G2] + self + real).           */
+void orc_spme_full_radii(double xi, int K, int p, double rc, int N, double L,
+                         const double *pos, const double *rad, int nrhs, double alpha,
+                         const double *vec_in, int ldin, double beta, double *vec_out,
+                         int ldout)
+{
+    size_t K3 = (size_t)K * K * K, Kc = (size_t)K * K * (K / 2 + 1);
+    int p3 = p * p * p, Kh = K / 2 + 1, nn, c, j;
+    size_t n;
+    double *P = (double *)malloc(sizeof(double) * (size_t)N * p3);
+    int *ind = (int *)malloc(sizeof(int) * (size_t)N * p3);
+    double *grid = (double *)malloc(sizeof(double) * 6 * K3);
+    double *spec = (double *)malloc(sizeof(double) * 6 * 2 * Kc);
+    double *lmA = (double *)malloc(sizeof(double) * Kc);
+    double *f2 = (double *)malloc(sizeof(double) * 3 * (size_t)N);
+    double *t2 = (double *)malloc(sizeof(double) * 3 * (size_t)N);
+    double *map = (double *)malloc(sizeof(double) * K);
+    double *b = (double *)malloc(sizeof(double) * 2 * K);
+    double a2 = alpha * ((double)K * K * K) / (L * L * L);
+    orc_compute_map(K, L, map);
+    orc_compute_b(K, p, L, map, b);
+    /* A |b_x b_y b_z|^2 (lm2 of compute_spline with zero radii) */
+#pragma omp parallel for schedule(static)
+    for (j = 0; j < K * K; j++) {
+        int z = j / K, y = j % K, x;
+        for (x = 0; x <= K / 2; x++) {
+            double kx = map[x], ky = map[y], kz = map[z];
+            double kk = sqrt(kx * kx + ky * ky + kz * kz), tr, ti, br, bi;
+            if (kk == 0.0) { lmA[(size_t)j * Kh + x] = 0.0; continue; }
+            tr = b[2 * x] * b[2 * y] - b[2 * x + 1] * b[2 * y + 1];
+            ti = b[2 * x] * b[2 * y + 1] + b[2 * x + 1] * b[2 * y];
+            br = tr * b[2 * z] - ti * b[2 * z + 1];
+            bi = tr * b[2 * z + 1] + ti * b[2 * z];
+            lmA[(size_t)j * Kh + x] = orc_scalar_recip(kk, xi, 0.0, 0.0) * (br * br + bi * bi);
+        }
+    }
+    orc_compute_P(K, p, pos, N, L, P, ind, NULL, NULL);
+    for (nn = 0; nn < nrhs; nn++) {
+        const double *f = vec_in + (size_t)nn * ldin;
+        double *u = vec_out + (size_t)nn * ldout;
+        for (n = 0; n < (size_t)N; n++)
+            for (c = 0; c < 3; c++) f2[3 * n + c] = rad[n] * rad[n] * f[3 * n + c];
+        orc_spread(K, p, N, P, ind, f, grid);
+        orc_spread(K, p, N, P, ind, f2, grid + 3 * K3);
+        for (c = 0; c < 6; c++) orc_fft3d_r2c(K, grid + c * K3, spec + c * 2 * Kc);
+        /* G1 = A P (S_f - k^2/6 S_a), G2 = (k^2/6) A P S_f, in place */
+#pragma omp parallel for schedule(static)
+        for (j = 0; j < K * K; j++) {
+            int z = j / K, y = j % K, x, h, m;
+            double kz = map[z], ky = map[y];
+            for (x = 0; x <= K / 2; x++) {
+                size_t o = (size_t)j * Kh + x;
+                double kx = map[x], k2 = kx * kx + ky * ky + kz * kz, kv[3] = {kx, ky, kz};
+                if (k2 == 0.0) {
+                    for (c = 0; c < 6; c++) spec[c * 2 * Kc + 2 * o] = spec[c * 2 * Kc + 2 * o + 1] = 0.0;
+                    continue;
+                }
+                for (h = 0; h < 2; h++) {
+                    double g1[3], g2[3], d1 = 0.0, d2 = 0.0;
+                    for (m = 0; m < 3; m++) {
+                        double sf = spec[m * 2 * Kc + 2 * o + h];
+                        double sa = spec[(3 + m) * 2 * Kc + 2 * o + h];
+                        g1[m] = sf - k2 / 6.0 * sa;
+                        g2[m] = sf;
+                        d1 += g1[m] * kv[m];
+                        d2 += g2[m] * kv[m];
+                    }
+                    for (m = 0; m < 3; m++) {
+                        spec[m * 2 * Kc + 2 * o + h] = lmA[o] * (g1[m] - d1 * kv[m] / k2);
+                        spec[(3 + m) * 2 * Kc + 2 * o + h] =
+                            lmA[o] * k2 / 6.0 * (g2[m] - d2 * kv[m] / k2);
+                    }
+                }
+            }
+        }
+        for (c = 0; c < 6; c++) {
+            orc_fft3d_c2r(K, spec + c * 2 * Kc, grid + c * K3);
+            for (n = 0; n < K3; n++) grid[c * K3 + n] *= 1.0 / ((double)K * K * K);
+        }
+        orc_interpolate(K, p, N, P, ind, grid, a2, beta, u);
+        orc_interpolate(K, p, N, P, ind, grid + 3 * K3, a2, 0.0, t2);
+        for (n = 0; n < (size_t)N; n++)
+            for (c = 0; c < 3; c++) u[3 * n + c] -= rad[n] * rad[n] * t2[3 * n + c];
+        for (n = 0; n < (size_t)N; n++) {
+            double sc = orc_self_coeff_radius(xi, rad[n]);
+            for (c = 0; c < 3; c++) t2[3 * n + c] = sc * f[3 * n + c];
+        }
+        orc_real_space_radii(N, pos, rad, L, xi, rc, f, N, NULL, t2);
+        for (n = 0; n < 3 * (size_t)N; n++) u[n] += alpha * t2[n];
+    }
+    free(P); free(ind); free(grid); free(spec); free(lmA); free(f2); free(t2); free(map);
+    free(b);
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

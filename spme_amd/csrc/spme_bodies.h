@@ -854,7 +854,12 @@ __device__ __forceinline__ void fft_y_block(T *__restrict__ grid, long comp_stri
 // N2-point FFT, the 3x3 influence and the backward N2-point FFT.
 // nyq: the Nyquist column as [component][z][y] (its blocks then read 16
 // consecutive y per z), or null when it sits in the spectrum rows.
-template <typename T, int K, int XW, int NT>
+// POLY (unequal radii): a block takes the 6 components of a right-hand side,
+// the spectra S_f (components 0-2) and S_{a^2 f} (3-5), and tab holds the
+// factors of A(k) = psi(k) without its radius term, so that
+//   out_f = A|b|^2 P (S_f - k^2/6 S_{a^2 f}),   out_a = (k^2/6) A|b|^2 P S_f
+// (P = I - kk^T/k^2); interpolation then forms I[out_f] - a_i^2 I[out_a].
+template <typename T, int K, int XW, int NT, bool POLY = false>
 __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp_stride,
                                                  int Khp, const T *__restrict__ tab, T dk,
                                                  const cpx<T> *__restrict__ tw, int lb,
@@ -863,6 +868,7 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
   constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
   constexpr int Kh = K / 2 + 1;
   constexpr int LS = K + 1;
+  constexpr int NCP = POLY ? 6 : 3;  // components per block
   using Z = ZLayout<XW>;
   static_assert(XW * N1 <= NT, "one step-B column per thread");
   constexpr int NXC = Z::chunks(K), PER = Z::per_rhs(K);
@@ -876,14 +882,14 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
   } else {
     xa = K / 2, ya = (j - NXC * K) * XW, dx = 0, dy = 1;
   }
-  cpx<T> *g0 = reinterpret_cast<cpx<T> *>(grid + (3L * rhs) * comp_stride);
+  cpx<T> *g0 = reinterpret_cast<cpx<T> *>(grid + ((long)NCP * rhs) * comp_stride);
   const long cs = comp_stride / 2;  // complex elements
   // this block's column values in the separate Nyquist array (block-uniform)
-  cpx<T> *nq = (nyq && j >= NXC * K) ? nyq + 3L * rhs * K * K : nullptr;
+  cpx<T> *nq = (nyq && j >= NXC * K) ? nyq + (long)NCP * rhs * K * K : nullptr;
   constexpr long NQC = (long)K * K;  // Nyquist values per component
-  for (int t = threadIdx.x; t < 3 * XW * N2; t += NT) {  // step A (forward) from global
+  for (int t = threadIdx.x; t < NCP * XW * N2; t += NT) {  // step A (forward) from global
     const int xo = t % XW, r = t / XW;
-    const int c = r % 3, n2 = r / 3;
+    const int c = r % NCP, n2 = r / NCP;
     const int x = xa + xo * dx, y = ya + xo * dy;
     cpx<T> a[N1];
 #pragma unroll
@@ -913,9 +919,9 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
       const int z = k1 + N1 * k2, nz = z <= K / 2 ? z : K - z;
       m2[k2] = tab[nxy + nz * nz] * (bxy * bsq[z]);
     }
-    cpx<T> b[3][N2];
+    cpx<T> b[NCP][N2];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < NCP; ++c) {
 #pragma unroll
       for (int n2 = 0; n2 < N2; ++n2) b[c][n2] = buf[(c * XW + xo) * LS + k1 * N2 + n2];
       reg_fft<N2, -1>(b[c]);
@@ -935,7 +941,22 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
       const T ax = mi * kx, ay = mi * ky, az = mi * kz;
       const T B0 = fma(-ax, kx, m2[k2]), B1 = -ax * ky, B2 = -ax * kz;
       const T B3 = fma(-ay, ky, m2[k2]), B4 = -ay * kz, B5 = fma(-az, kz, m2[k2]);
-      const cpx<T> va = b[0][k2], vb = b[1][k2], vc = b[2][k2];
+      cpx<T> va = b[0][k2], vb = b[1][k2], vc = b[2][k2];  // S_f
+      if constexpr (POLY) {
+        // out_a = (k^2/6) B S_f into components 3-5; out_f = B (S_f - k^2/6
+        // S_{a^2 f}) below, from the combination formed in va, vb, vc
+        const T q = k2n * T(1.0 / 6.0);
+        const cpx<T> sa = b[3][k2], sb = b[4][k2], sc = b[5][k2];  // S_{a^2 f}
+        b[3][k2] = {q * fma(B2, vc.x, fma(B1, vb.x, B0 * va.x)),
+                    q * fma(B2, vc.y, fma(B1, vb.y, B0 * va.y))};
+        b[4][k2] = {q * fma(B4, vc.x, fma(B3, vb.x, B1 * va.x)),
+                    q * fma(B4, vc.y, fma(B3, vb.y, B1 * va.y))};
+        b[5][k2] = {q * fma(B5, vc.x, fma(B4, vb.x, B2 * va.x)),
+                    q * fma(B5, vc.y, fma(B4, vb.y, B2 * va.y))};
+        va = {fma(-q, sa.x, va.x), fma(-q, sa.y, va.y)};
+        vb = {fma(-q, sb.x, vb.x), fma(-q, sb.y, vb.y)};
+        vc = {fma(-q, sc.x, vc.x), fma(-q, sc.y, vc.y)};
+      }
       b[0][k2] = {fma(B2, vc.x, fma(B1, vb.x, B0 * va.x)),
                   fma(B2, vc.y, fma(B1, vb.y, B0 * va.y))};
       b[1][k2] = {fma(B4, vc.x, fma(B3, vb.x, B1 * va.x)),
@@ -944,7 +965,7 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
                   fma(B5, vc.y, fma(B4, vb.y, B2 * va.y))};
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+    for (int c = 0; c < NCP; ++c) {
       reg_fft<N2, +1>(b[c]);
 #pragma unroll
       for (int n2 = 1; n2 < N2; ++n2) b[c][n2] = cmul(b[c][n2], twiddle<+1>(s_tw, n2 * k1));
@@ -953,9 +974,9 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
     }
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < 3 * XW * N2; t += NT) {  // inverse step A to global
+  for (int t = threadIdx.x; t < NCP * XW * N2; t += NT) {  // inverse step A to global
     const int xo = t % XW, r = t / XW;
-    const int c = r % 3, n2 = r / 3;
+    const int c = r % NCP, n2 = r / NCP;
     const int x = xa + xo * dx, y = ya + xo * dy;
     cpx<T> a[N1];
 #pragma unroll
@@ -981,7 +1002,9 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
 //                                  + alpha*(self f + sum_j B_sj f_j),
 // B_sj = F I + (G/r^2) e e^T with e the minimum-image separation, formed
 // from the stored FP64-built F and G/r^2 and the FP64 positions (8 stored
-// bytes per entry instead of 6 block values).
+// bytes per entry instead of 6 block values).  With unequal radii the
+// stored F and G carry the pair's radii and the self coefficient is per row
+// (selfv, cell order; null: the scalar self).
 // Rows are packed in slices of 64 (entry-major within a slice); the rows of
 // a wave lie in one slice, so every lane runs the same entry count.
 template <int C> struct RealShape {
@@ -1024,6 +1047,7 @@ __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
                                           const int *__restrict__ col,
                                           const T *__restrict__ blk, long E,
                                           const double *__restrict__ spos, double L, T self,
+                                          const T *__restrict__ selfv,
                                           const T *__restrict__ fs, TO alpha, TO beta, int accum,
                                           TO *__restrict__ out, long ldout, int s, int l) {
   constexpr int RPL = RealShape<C>::RPL, NV = 3 * RPL;
@@ -1031,32 +1055,37 @@ __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
   const int r0 = l * RPL;
   T a[NV], f[NV];
   load_n<T, NV>(fs + (size_t)s * C + 3 * r0, f);
+  // self term: one coefficient, or one per row (unequal radii)
+  const T sf = selfv ? selfv[s] : self;
 #pragma unroll
-  for (int i = 0; i < NV; ++i) a[i] = self * f[i];
+  for (int i = 0; i < NV; ++i) a[i] = sf * f[i];
   const double px = spos[3L * s], py = spos[3L * s + 1], pz = spos[3L * s + 2];
   const double iL = 1.0 / L;
   const long base = soff[q] + ls;
   const int w = width[q];
 #pragma unroll 4
   for (int k = 0; k < w; ++k) {
+    // every fused multiply-add is written out, so that the instantiations
+    // for different batch widths (right-hand sides per lane) round alike
+#pragma clang fp contract(off)
     const long e = base + 64L * k;
     const int j = col[e];
     const T F = blk[e], g = blk[E + e];
     double ex = spos[3L * j] - px, ey = spos[3L * j + 1] - py, ez = spos[3L * j + 2] - pz;
-    ex -= L * rint(ex * iL);
-    ey -= L * rint(ey * iL);
-    ez -= L * rint(ez * iL);
+    ex = fma(-L, rint(ex * iL), ex);
+    ey = fma(-L, rint(ey * iL), ey);
+    ez = fma(-L, rint(ez * iL), ez);
     const T x = T(ex), y = T(ey), z = T(ez);
     const T gx = g * x, gy = g * y, gz = g * z;
-    const T b0 = F + gx * x, b1 = gx * y, b2 = gx * z;
-    const T b3 = F + gy * y, b4 = gy * z, b5 = F + gz * z;
+    const T b0 = fma(gx, x, F), b1 = gx * y, b2 = gx * z;
+    const T b3 = fma(gy, y, F), b4 = gy * z, b5 = fma(gz, z, F);
     load_n<T, NV>(fs + (size_t)j * C + 3 * r0, f);
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
       const T f0 = f[3 * r], f1 = f[3 * r + 1], f2 = f[3 * r + 2];
-      a[3 * r] += b0 * f0 + b1 * f1 + b2 * f2;
-      a[3 * r + 1] += b1 * f0 + b3 * f1 + b4 * f2;
-      a[3 * r + 2] += b2 * f0 + b4 * f1 + b5 * f2;
+      a[3 * r] = fma(b2, f2, fma(b1, f1, fma(b0, f0, a[3 * r])));
+      a[3 * r + 1] = fma(b4, f2, fma(b3, f1, fma(b1, f0, a[3 * r + 1])));
+      a[3 * r + 2] = fma(b5, f2, fma(b4, f1, fma(b2, f0, a[3 * r + 2])));
     }
   }
   const long po = 3L * perm[s];
@@ -1065,8 +1094,9 @@ __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
     TO *u = out + (r0 + r) * ldout + po;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
+#pragma clang fp contract(off)
       const TO prev = accum ? u[d] : (beta == TO(0) ? TO(0) : beta * u[d]);
-      u[d] = prev + alpha * TO(a[3 * r + d]);
+      u[d] = fma(alpha, TO(a[3 * r + d]), prev);
     }
   }
 }

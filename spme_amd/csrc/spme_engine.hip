@@ -182,6 +182,15 @@ struct _spme_t {
   spme_tiles_t tiles;
   unsigned pos_version;
   int ncu;  // compute units of the device
+  // unequal radii (prm.radii != 0): the mesh carries Cm = 6 components per
+  // right-hand side (f and a^2 f), rad the radii (original order, 1 by
+  // default), r_rad in cell order, selfv the per-row self coefficients,
+  // tmp the interpolated influence outputs [2 nrhs][3N] before combining
+  int poly, Cm;
+  double *rad, *r_rad;
+  void *selfv, *tmp;
+  double cut;    // pair search radius: max(rc, 2 a_max), at least 2
+  int nc_alloc;  // cells per side the cell arrays were sized for
 };
 
 namespace {
@@ -204,7 +213,9 @@ int upload_tables(_spme_t *e) {
   } else {
     // the z pass forms psi |b|^2 / V from its factors
     HIP_TRY(hipMalloc(&e->tab, sizeof(T) * (size_t)(influence_npsi(K) + K)));
-    launch_influence_factors<T>(d_bsq, K, 2.0 * M_PI / L, xi, 1.0 / V, (T *)e->tab, e->stream);
+    // unequal radii: the factors of A(k), psi without its radius term
+    launch_influence_factors<T>(d_bsq, K, 2.0 * M_PI / L, xi, 1.0 / V, e->poly, (T *)e->tab,
+                                e->stream);
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(e->stream));
@@ -238,6 +249,13 @@ int validate(const spme_params_t *p) {
     return fail(-1, "precision must be SPME_FP64, SPME_FP32, SPME_FP64_PLAIN or "
                     "SPME_FP32_SUM64");
   if (p->max_rhs < 1 || p->max_rhs > 16) return fail(-1, "max_rhs must be in [1, 16]");
+  if (p->radii) {
+    if (p->precision == SPME_FP64_PLAIN)
+      return fail(-1, "unequal radii are not available on the plain path");
+    if (p->max_rhs > 8)
+      return fail(-1, "with unequal radii max_rhs must be in [1, 8] (six mesh components per "
+                      "right-hand side)");
+  }
   // the spreading stages particle data by 32-bit dword offsets into the
   // sorted force and weight arrays (at most N * 48 values of 2 dwords)
   if (p->np > (1 << 23)) return fail(-1, "np must be at most %d (got %d)", 1 << 23, p->np);
@@ -249,7 +267,8 @@ void free_all(_spme_t *e) {
                   e->cursor, e->boff, e->perm, e->pbase, e->err, e->w_tmp, e->pw, e->fs,
                   e->r_key, e->r_counts, e->r_cursor, e->r_off, e->r_perm, e->r_pos,
                   e->r_fs, e->r_cnt, e->r_width, e->r_size, e->r_soff, e->r_col, e->r_blk,
-                  e->h_in, e->h_out, e->dP, e->dind, e->iperm, e->r_iperm, e->bigq};
+                  e->h_in, e->h_out, e->dP, e->dind, e->iperm, e->r_iperm, e->bigq,
+                  e->rad, e->r_rad, e->selfv, e->tmp};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &r : e->pending) {
@@ -278,7 +297,10 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
   e->comp_stride = (long)e->K * e->K * e->RS;
   // the mesh path runs in FP32 for SPME_FP32 and SPME_FP32_SUM64
   e->tsz = (e->prec == SPME_FP32 || e->prec == SPME_FP32_SUM64) ? 4 : 8;
-  const int K = e->K, N = e->N, C = 3 * e->max_rhs;
+  e->poly = prm->radii ? 1 : 0;
+  e->Cm = e->poly ? 6 : 3;
+  // C: mesh components; the real-space forces keep 3 per right-hand side
+  const int K = e->K, N = e->N, C = e->Cm * e->max_rhs;
   auto bail = [&](int rc) {
     free_all(e);
     delete e;
@@ -327,9 +349,16 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
   // Real-space cell list only when a cutoff is given (the drop-in surface is
   // reciprocal-only).  Cells are at least max(rc, 2) wide so that every pair
   // within the cutoff or overlapping lies in adjacent cells.
-  const double cell = prm->rc > 2.0 ? prm->rc : 2.0;
-  e->nc = prm->rc > 0.0 ? int(std::floor(L / cell)) : 0;
+  e->cut = prm->rc > 2.0 ? prm->rc : 2.0;
+  e->nc = prm->rc > 0.0 ? int(std::floor(L / e->cut)) : 0;
   if (e->nc > 128) e->nc = 128;
+  e->nc_alloc = e->nc;
+  if (e->poly) {
+    std::vector<double> ones(N, 1.0);
+    E_TRY(hipMalloc(&e->rad, sizeof(double) * N));
+    E_TRY(hipMemcpy(e->rad, ones.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    E_TRY(hipMalloc(&e->tmp, e->tsz * (size_t)C * N));
+  }
   if (e->nc >= 3) {
     const int nc3 = e->nc * e->nc * e->nc;
     E_TRY(hipMalloc(&e->r_key, sizeof(int) * N));
@@ -339,12 +368,16 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
     E_TRY(hipMalloc(&e->r_perm, sizeof(int) * N));
     E_TRY(hipMalloc(&e->r_iperm, sizeof(int) * N));
     E_TRY(hipMalloc(&e->r_pos, sizeof(double) * 3 * N));
-    E_TRY(hipMalloc(&e->r_fs, e->tsz * (size_t)C * N));
+    E_TRY(hipMalloc(&e->r_fs, e->tsz * (size_t)3 * e->max_rhs * N));
     e->nslices = (N + 63) / 64;
     E_TRY(hipMalloc(&e->r_cnt, sizeof(int) * N));
     E_TRY(hipMalloc(&e->r_width, sizeof(int) * e->nslices));
     E_TRY(hipMalloc(&e->r_size, sizeof(int) * e->nslices));
     E_TRY(hipMalloc(&e->r_soff, sizeof(int) * (e->nslices + 1)));
+    if (e->poly) {
+      E_TRY(hipMalloc(&e->r_rad, sizeof(double) * N));
+      E_TRY(hipMalloc(&e->selfv, e->tsz * N));
+    }
   }
 #undef E_TRY
   *out = e;
@@ -383,9 +416,10 @@ int build_P(_spme_t *e, hipStream_t st) {
     if (launch_bucket_sort(e->r_key, N, e->r_off, e->r_perm, nc3, e->bigq, e->r_iperm, st))
       return fail(-2, "cell sort launch failed");
     launch_invert(e->r_perm, e->r_iperm, N, st);
-    launch_cell_gather(e->r_perm, e->pos, e->r_pos, N, st);
+    launch_cell_gather(e->r_perm, e->pos, e->r_pos, e->poly ? e->rad : nullptr, e->r_rad, N, st);
+    if (e->poly) launch_self_radii<T>(e->r_rad, N, e->prm.xi, (T *)e->selfv, st);
     const double L = e->prm.box_size, rc = e->prm.rc;
-    const double cut = rc > 2.0 ? rc : 2.0;
+    const double cut = e->cut;
     launch_pair_count(e->r_pos, e->r_off, e->nc, L, cut * cut, N, e->r_cnt, st);
     launch_slice_width(e->r_cnt, N, e->r_width, e->r_size, st);
     // the entry offsets are ints: sum the slice sizes in 64 bits first and
@@ -416,7 +450,8 @@ int build_P(_spme_t *e, hipStream_t st) {
     e->r_E = e->r_cap;
     e->r_used = total;
     launch_pair_fill<T>(e->r_pos, e->r_off, e->nc, L, e->prm.xi, rc * rc, cut * cut, N,
-                        e->r_width, e->r_soff, e->r_col, (T *)e->r_blk, e->r_E, e->err, st);
+                        e->r_width, e->r_soff, e->r_col, e->poly ? e->r_rad : nullptr,
+                        (T *)e->r_blk, e->r_E, e->err, st);
   }
   HIP_TRY(hipGetLastError());
   return 0;
@@ -441,6 +476,43 @@ int set_positions(_spme_t *e, const double *pos, int on_device, hipStream_t st) 
   if (herr & 2)
     return fail(-2, "real-space block list: a row found more neighbours than its count pass");
   e->have_pos = true;
+  return 0;
+}
+
+// Radii (N values, FP64, > 0) for an engine created with radii: kept in
+// original order; the pair search radius becomes max(rc, 2, 2 a_max), so
+// the cells may grow (never beyond the arrays sized at create).  The P data
+// and block list depend on the radii: positions must be set again.
+int set_radii(_spme_t *e, const double *rad, int on_device, hipStream_t st) {
+  if (!e->poly) return fail(-1, "the engine was created without radii (spme_params_t.radii)");
+  if (!rad) return fail(-1, "radii are NULL");
+  e->have_pos = false;
+  ++e->pos_version;
+  std::vector<double> h(e->N);
+  if (on_device) {
+    HIP_TRY(hipMemcpyAsync(h.data(), rad, sizeof(double) * e->N, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  } else {
+    std::copy(rad, rad + e->N, h.begin());
+  }
+  double amax = 0.0;
+  for (double a : h) {
+    if (!(a > 0.0) || !std::isfinite(a)) return fail(-1, "radii must be positive and finite");
+    amax = std::max(amax, a);
+  }
+  const double rc = e->prm.rc;
+  const double cut = std::max(std::max(rc, 2.0), 2.0 * amax);
+  if (e->nc_alloc >= 3) {
+    int nc = int(std::floor(e->prm.box_size / cut));
+    nc = std::min(nc, e->nc_alloc);
+    if (nc < 3)
+      return fail(-1, "radii up to %g need box_size >= 3 * max(rc, 2 a_max) = %g for the "
+                      "real-space part", amax, 3.0 * cut);
+    e->nc = nc;
+  }
+  e->cut = cut;
+  HIP_TRY(hipMemcpyAsync(e->rad, h.data(), sizeof(double) * e->N, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
   return 0;
 }
 
@@ -557,21 +629,24 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
       mark(SPME_ST_INTERP);
     }
   }
-  const Tiles tl = resolve_tiles(e, e->tiles, C);
+  // mesh components: 3 per right-hand side, 6 with unequal radii (f, a^2 f)
+  const int Cv = e->Cm * nrhs;
+  const Tiles tl = resolve_tiles(e, e->tiles, Cv);
   // one pass writes the forces in both sorted orders when both are needed
   const bool both = (parts & SPME_RECIP) && !e->dense && (parts & SPME_REAL);
   if ((parts & SPME_RECIP) && !e->dense) {
     launch_permute_in<T>(e->iperm, (T *)e->fs, both ? e->r_iperm : nullptr,
-                         both ? (T *)e->r_fs : nullptr, in, ldin, N, C, st);
+                         both ? (T *)e->r_fs : nullptr, in, ldin, N, C, st,
+                         e->poly ? e->rad : nullptr);
     mark(SPME_ST_PERMUTE);
-    SpreadArgs<T> sa{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, C, tl.spread_cc,
+    SpreadArgs<T> sa{e->boff, e->pbase, (const T *)e->pw, (const T *)e->fs, Cv, tl.spread_cc,
                      e->P, (T *)e->grid, e->comp_stride, e->RS, e->K, tl.spread_zc};
     if (launch_spread<T>(sa, st)) return fail(-1, "spread: unsupported configuration");
     mark(SPME_ST_SPREAD);
     if (stop_after == 1) return 0;
-    FftArgs<T> fa{(T *)e->grid, e->comp_stride, e->RS, e->K, C, (const T *)e->tab,
+    FftArgs<T> fa{(T *)e->grid, e->comp_stride, e->RS, e->K, Cv, (const T *)e->tab,
                   T(2.0 * M_PI / e->prm.box_size), (const cpx<T> *)e->tw, e->ncu,
-                  (cpx<T> *)e->nyq};
+                  (cpx<T> *)e->nyq, e->poly};
     FftMarks fm{};
     if (timed)
       for (int i = 0; i < 5; ++i) fm.ev[i] = take_event(e);
@@ -582,10 +657,21 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
       for (int i = fm.n; i < 5; ++i) e->pool.push_back(fm.ev[i]);
     }
     if (stop_after == 2) return 0;
-    InterpArgs<T, TO> ia{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid,
-                         e->comp_stride, e->RS, e->K, N, C, e->P, tl.interp_zc, TO(alpha),
-                         TO(beta), out, ldout};
-    if (launch_interp(ia, st)) return fail(-1, "interpolate: unsupported configuration");
+    if (e->poly) {
+      // out_f and out_a of every right-hand side into tmp, then
+      // out = beta out + alpha (I[out_f] - a_i^2 I[out_a])
+      InterpArgs<T, T> ia{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid,
+                          e->comp_stride, e->RS, e->K, N, Cv, e->P, tl.interp_zc, T(1), T(0),
+                          (T *)e->tmp, 3L * N};
+      if (launch_interp(ia, st)) return fail(-1, "interpolate: unsupported configuration");
+      launch_combine_radii<T, TO>((const T *)e->tmp, e->rad, N, nrhs, alpha, beta, out, ldout,
+                                  st);
+    } else {
+      InterpArgs<T, TO> ia{e->boff, e->pbase, (const T *)e->pw, e->perm, (const T *)e->grid,
+                           e->comp_stride, e->RS, e->K, N, C, e->P, tl.interp_zc, TO(alpha),
+                           TO(beta), out, ldout};
+      if (launch_interp(ia, st)) return fail(-1, "interpolate: unsupported configuration");
+    }
     mark(SPME_ST_INTERP);
   }
   if (parts & SPME_REAL) {
@@ -599,7 +685,8 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
                                   40.0 * xi * xi * xi / (3.0 * std::sqrt(M_PI))
                             : 0.0;
     RealArgs<T, TO> ra{e->r_perm, e->r_width, e->r_soff, e->r_col, (const T *)e->r_blk, e->r_E,
-                   e->r_pos, e->prm.box_size, self, (const T *)e->r_fs, N, C, alpha, beta,
+                   e->r_pos, e->prm.box_size, self, e->poly ? (const T *)e->selfv : nullptr,
+                   (const T *)e->r_fs, N, C, alpha, beta,
                    (parts & SPME_RECIP) ? 1 : 0, out, ldout};
     if (launch_real(ra, st)) return fail(-1, "real space: unsupported batch width");
     mark(SPME_ST_REAL);
@@ -683,7 +770,7 @@ std::vector<int> zc_candidates(int K, int lo, int hi) {
 
 template <typename T, typename TO>
 int autotune_t(_spme_t *e, int nrhs, int reps, hipStream_t st) {
-  const int C = 3 * nrhs, K = e->K, N = e->N;
+  const int C = e->Cm * nrhs, K = e->K, N = e->N;  // mesh components
   EventPair ev;
   if (int rc = ev.init()) return rc;
   DevBuf out;
@@ -888,7 +975,8 @@ int group_build_args(_spme_group_t *g, double alpha, const void *const *in, cons
                                                   40.0 * xi * xi * xi / (3.0 * std::sqrt(M_PI))
                                             : 0.0;
     ra[s] = RealArgs<T>{e->r_perm, e->r_width, e->r_soff, e->r_col, (const T *)e->r_blk,
-                        e->r_E, e->r_pos, e->prm.box_size, self, (const T *)e->r_fs, e->N, C,
+                        e->r_E, e->r_pos, e->prm.box_size, self, nullptr, (const T *)e->r_fs,
+                        e->N, C,
                         alpha, beta, rec ? 1 : 0, (T *)out[s], ldout[s]};
   }
   HIP_TRY(hipMemcpyAsync(g->d_args, g->h_args, bytes, hipMemcpyHostToDevice, st));
@@ -1151,6 +1239,11 @@ int spme_engine_set_positions(spme_t *spme, const double *pos, int on_device, vo
   return set_positions(spme, pos, on_device, (hipStream_t)stream);
 }
 
+int spme_engine_set_radii(spme_t *spme, const double *radii, int on_device, void *stream) {
+  if (!spme) return fail(-1, "null engine");
+  return set_radii(spme, radii, on_device, (hipStream_t)stream);
+}
+
 int spme_engine_apply(spme_t *spme, int nrhs, double alpha, const void *in, long ldin,
                       double beta, void *out, long ldout, int parts, void *stream) {
   if (!spme) return fail(-1, "null engine");
@@ -1202,6 +1295,7 @@ int spme_engine_debug_stage(spme_t *e, int stage, int nrhs, const void *in, long
   if (!e) return fail(-1, "null engine");
   if (stage != 1 && stage != 2) return fail(-1, "stage must be 1 or 2");
   if (e->dense) return fail(-1, "debug stages are not available on the plain path");
+  if (e->poly) return fail(-1, "debug stages are not available with unequal radii");
   int rc = apply(e, nrhs, 1.0, in, ldin, 0.0, (void *)in, ldin, SPME_RECIP, e->stream, stage);
   if (rc) return rc;
   HIP_TRY(hipMemcpyAsync(host_mesh, e->grid, (size_t)3 * nrhs * e->comp_stride * e->tsz,
@@ -1289,6 +1383,7 @@ int spme_group_create(int nsys, spme_t *const *engines, int nrhs, spme_group_t *
     if (e->dense) return fail(-1, "engine %d: plain-path engines cannot be grouped", s);
     if (e->prec == SPME_FP32_SUM64)
       return fail(-1, "engine %d: groups take SPME_FP32 or SPME_FP64 engines", s);
+    if (e->poly) return fail(-1, "engine %d: engines with unequal radii cannot be grouped", s);
     if (e->prec != engines[0]->prec || e->P != engines[0]->P)
       return fail(-1, "engine %d: all engines need one precision and one porder", s);
     if (nrhs < 1 || nrhs > e->max_rhs)

@@ -84,8 +84,8 @@ __global__ __launch_bounds__(GROUP_ELEMS) void kg_real(GroupStage<RealArgs<T>> t
     const long gi = (long)lb * GROUP_ELEMS + threadIdx.x;
     const int row = int(gi / LPR), l = int(gi % LPR);
     if (row < a.N)
-      real_lane<T, C>(a.perm, a.width, a.soff, a.col, a.blk, a.E, a.spos, a.L, T(a.self), a.fs,
-                      T(a.alpha), T(a.beta), a.accum, a.out, a.ldout, row, l);
+      real_lane<T, C>(a.perm, a.width, a.soff, a.col, a.blk, a.E, a.spos, a.L, T(a.self),
+                      a.selfv, a.fs, T(a.alpha), T(a.beta), a.accum, a.out, a.ldout, row, l);
   });
 }
 

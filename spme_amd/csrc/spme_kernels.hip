@@ -171,6 +171,9 @@ __global__ __launch_bounds__(256) void k_pgather(const int *__restrict__ perm,
 
 // Forces into sorted particle-major order, one or two orders in one pass:
 // fa[ia[i]*C + 3r + d] = in[r*ldin + 3i + d] (and fb through ib), C <= 48.
+// With radii (rad != null, unequal-radius engines, C <= 24) fa takes six
+// components per right-hand side, f and a_i^2 f:
+// fa[ia[i]*2C + 6r + d] = f, fa[ia[i]*2C + 6r + 3 + d] = a_i^2 f.
 // A block stages PB consecutive original particles in LDS (each input row
 // read once, coalesced), then writes each particle's C values as one
 // contiguous record per order.
@@ -185,7 +188,7 @@ __global__ __launch_bounds__(256) void k_permute_in(const int *__restrict__ ia,
                                                     const int *__restrict__ ib,
                                                     T *__restrict__ fb,
                                                     const T *__restrict__ in, long ldin, int N,
-                                                    int C) {
+                                                    int C, const double *__restrict__ rad) {
   constexpr int PB = PermuteShape<T>::PB, CP = PermuteShape<T>::CP;
   __shared__ T s[PB * CP];
   const int i0 = blockIdx.x * PB, n = min(PB, N - i0), tid = threadIdx.x;
@@ -200,7 +203,15 @@ __global__ __launch_bounds__(256) void k_permute_in(const int *__restrict__ ia,
   for (int e = tid; e < n * C; e += 256) {
     const int i = e / C, c = e - i * C;
     const T v = s[i * CP + c];
-    fa[(size_t)ia[i0 + i] * C + c] = v;
+    if (rad) {
+      const int r = c / 3, d = c - 3 * r;
+      const double a = rad[i0 + i];
+      T *o = fa + (size_t)ia[i0 + i] * (2 * C) + 6 * r + d;
+      o[0] = v;
+      o[3] = T(a * a) * v;
+    } else {
+      fa[(size_t)ia[i0 + i] * C + c] = v;
+    }
     if (fb) fb[(size_t)ib[i0 + i] * C + c] = v;
   }
 }
@@ -275,15 +286,15 @@ __global__ __launch_bounds__(NT) void k_fft_z(T *__restrict__ grid, long comp_st
                                      xcd_remap(blockIdx.x, gridDim.x), buf, s_tw);
 }
 
-template <typename T, int K, int XW, int NT>
+template <typename T, int K, int XW, int NT, bool POLY = false>
 __global__ __launch_bounds__(NT) void k_fft_z_infl(T *__restrict__ grid, long comp_stride,
                                                    int Khp, const T *__restrict__ tab, T dk,
                                                    const cpx<T> *__restrict__ tw,
                                                    cpx<T> *__restrict__ nyq) {
-  __shared__ cpx<T> buf[3 * XW * (K + 1)];
+  __shared__ cpx<T> buf[(POLY ? 6 : 3) * XW * (K + 1)];
   __shared__ cpx<T> s_tw[K];
-  fft_z_infl_block<T, K, XW, NT>(grid, comp_stride, Khp, tab, dk, tw,
-                                 xcd_remap(blockIdx.x, gridDim.x), buf, s_tw, nyq);
+  fft_z_infl_block<T, K, XW, NT, POLY>(grid, comp_stride, Khp, tab, dk, tw,
+                                       xcd_remap(blockIdx.x, gridDim.x), buf, s_tw, nyq);
 }
 
 // ---------------------------------------------------------------------------
@@ -694,10 +705,13 @@ __global__ __launch_bounds__(256) void k_influence_table(const double *__restric
 // precision: fac[n] = psi(dk sqrt(n)) inv_v for n = nx^2 + ny^2 + nz^2 <=
 // 3 (K/2)^2 (the wave numbers are integer multiples of dk), then |b(m)|^2
 // for m < K; the z pass forms psi |b_x|^2 |b_y|^2 |b_z|^2 / V from them.
+// radius_free: A(k), psi without its radius term (1 - k^2 (a_i^2 + a_j^2)/6),
+// for unequal radii, which the z pass applies per pair of spectra.
 template <typename T>
 __global__ __launch_bounds__(256) void k_influence_factors(const double *__restrict__ bsq,
                                                            int K, double dk, double xi,
-                                                           double inv_v, T *__restrict__ fac) {
+                                                           double inv_v, int radius_free,
+                                                           T *__restrict__ fac) {
   const int npsi = influence_npsi(K);
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < npsi) {
@@ -705,8 +719,8 @@ __global__ __launch_bounds__(256) void k_influence_factors(const double *__restr
     double v = 0.0;
     if (t > 0) {
       const double q = k2 / (xi * xi);
-      v = 6.0 * M_PI * exp(-q / 4.0) * (1.0 - k2 / 3.0) / k2 * (1.0 + q * (0.25 + q / 8.0)) *
-          inv_v;
+      const double rterm = radius_free ? 1.0 : (1.0 - k2 / 3.0);
+      v = 6.0 * M_PI * exp(-q / 4.0) * rterm / k2 * (1.0 + q * (0.25 + q / 8.0)) * inv_v;
     }
     fac[t] = T(v);
   } else if (t < npsi + K) {
@@ -726,14 +740,14 @@ void launch_influence_table(const double *bsq, int K, int Khp, double dk, double
 }
 template <typename T>
 void launch_influence_factors(const double *bsq, int K, double dk, double xi, double inv_v,
-                              T *fac, hipStream_t st) {
+                              int radius_free, T *fac, hipStream_t st) {
   const int n = influence_npsi(K) + K;
   hipLaunchKernelGGL(k_influence_factors<T>, dim3((n + 255) / 256), dim3(256), 0, st, bsq, K, dk,
-                     xi, inv_v, fac);
+                     xi, inv_v, radius_free, fac);
 }
-template void launch_influence_factors<float>(const double *, int, double, double, double,
+template void launch_influence_factors<float>(const double *, int, double, double, double, int,
                                               float *, hipStream_t);
-template void launch_influence_factors<double>(const double *, int, double, double, double,
+template void launch_influence_factors<double>(const double *, int, double, double, double, int,
                                                double *, hipStream_t);
 template void launch_influence_table<float>(const double *, int, int, double, double, double,
                                             float *, hipStream_t);
@@ -784,10 +798,42 @@ void launch_pgather(const int *perm, const int *base_tmp, const double *w_tmp, i
 
 template <typename T>
 void launch_permute_in(const int *ia, T *fa, const int *ib, T *fb, const T *in, long ldin, int N,
-                       int C, hipStream_t st) {
+                       int C, hipStream_t st, const double *rad) {
   hipLaunchKernelGGL(k_permute_in<T>, dim3(grid1(N, PermuteShape<T>::PB)), dim3(256), 0, st, ia,
-                     fa, ib, fb, in, ldin, N, C);
+                     fa, ib, fb, in, ldin, N, C, rad);
 }
+
+// Unequal radii: out[r*ldout + 3i + d] = beta*out + alpha*(g[2r][3i + d] -
+// a_i^2 g[2r + 1][3i + d]), the interpolated out_f and out_a of the
+// influence (each a vector of 3N values at stride 3N).
+template <typename T, typename TO>
+__global__ __launch_bounds__(256) void k_combine_radii(const T *__restrict__ g,
+                                                       const double *__restrict__ rad, int N,
+                                                       int nrhs, TO alpha, TO beta,
+                                                       TO *__restrict__ out, long ldout) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long n3 = 3L * N;
+  if (t >= n3 * nrhs) return;
+  const int r = int(t / n3);
+  const long e = t - r * n3;
+  const double a = rad[e / 3];
+  const TO v = TO(g[2 * r * n3 + e]) - TO(a * a) * TO(g[(2 * r + 1) * n3 + e]);
+  TO *u = out + r * ldout + e;
+  *u = beta == TO(0) ? alpha * v : beta * *u + alpha * v;
+}
+
+template <typename T, typename TO>
+void launch_combine_radii(const T *g, const double *rad, int N, int nrhs, double alpha,
+                          double beta, TO *out, long ldout, hipStream_t st) {
+  hipLaunchKernelGGL((k_combine_radii<T, TO>), dim3(grid1(3L * N * nrhs, 256)), dim3(256), 0, st,
+                     g, rad, N, nrhs, TO(alpha), TO(beta), out, ldout);
+}
+template void launch_combine_radii<float, float>(const float *, const double *, int, int,
+                                                 double, double, float *, long, hipStream_t);
+template void launch_combine_radii<float, double>(const float *, const double *, int, int,
+                                                  double, double, double *, long, hipStream_t);
+template void launch_combine_radii<double, double>(const double *, const double *, int, int,
+                                                   double, double, double *, long, hipStream_t);
 
 void launch_invert(const int *perm, int *inv, int N, hipStream_t st) {
   hipLaunchKernelGGL(k_invert, dim3(grid1(N, 256)), dim3(256), 0, st, perm, inv, N);
@@ -887,6 +933,18 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
   const int nxc = (K / 2 + 1 + S::XW - 1) / S::XW;
   using ZS = ZShape<T, K>;
   const int nzb = ZLayout<ZS::XW>::per_rhs(K) * (a.ncomp / 3);
+  // unequal radii: blocks of the 6 components of a right-hand side, in the
+  // narrow shape (twice the components per block)
+  constexpr int XWP = FftShape<T>::XWZ, NTP = 256;
+  const int nzbp = ZLayout<XWP>::per_rhs(K) * (a.ncomp / 6);
+  auto launch_z = [&](cpx<T> *nyq) {
+    if (a.poly)
+      hipLaunchKernelGGL((k_fft_z_infl<T, K, XWP, NTP, true>), dim3(nzbp), dim3(NTP), 0, st,
+                         a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw, nyq);
+    else
+      hipLaunchKernelGGL((k_fft_z_infl<T, K, ZS::XW, ZS::NT>), dim3(nzb), dim3(ZS::NT), 0, st,
+                         a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw, nyq);
+  };
   auto mark = [&](int pass) {
     if (mk) {
       (void)hipEventRecord(mk->ev[mk->n], st);
@@ -905,8 +963,7 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
       hipLaunchKernelGGL((k_fft_xy_fwd<T, K, NTXY, false>), dim3(grid), dim3(NTXY), 0, st,
                          a.grid, a.RS, a.comp_stride, nplanes, a.tw, a.nyq);
     mark(0);
-    hipLaunchKernelGGL((k_fft_z_infl<T, K, ZS::XW, ZS::NT>), dim3(nzb), dim3(ZS::NT), 0, st,
-                       a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw, a.nyq);
+    launch_z(a.nyq);
     mark(2);
     if (a.nyq)
       hipLaunchKernelGGL((k_fft_xy_inv<T, K, NTXY, true>), dim3(grid), dim3(NTXY), 0, st, a.grid,
@@ -922,8 +979,7 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
     hipLaunchKernelGGL((k_fft_y<T, K, S::XW, -1, NT>), dim3(nxc * K * a.ncomp), dim3(NT), 0,
                        st, a.grid, a.comp_stride, Khp, nxc, a.tw);
     mark(1);
-    hipLaunchKernelGGL((k_fft_z_infl<T, K, ZS::XW, ZS::NT>), dim3(nzb), dim3(ZS::NT), 0, st,
-                       a.grid, a.comp_stride, Khp, a.tab, a.dk, a.tw, nullptr);
+    launch_z(nullptr);
     mark(2);
     hipLaunchKernelGGL((k_fft_y<T, K, S::XW, +1, NT>), dim3(nxc * K * a.ncomp), dim3(NT), 0,
                        st, a.grid, a.comp_stride, Khp, nxc, a.tw);
@@ -1004,9 +1060,10 @@ template void launch_pgather<float>(const int *, const int *, const double *, in
 template void launch_pgather<double>(const int *, const int *, const double *, int *, double *,
                                      int, int, hipStream_t);
 template void launch_permute_in<float>(const int *, float *, const int *, float *, const float *,
-                                       long, int, int, hipStream_t);
+                                       long, int, int, hipStream_t, const double *);
 template void launch_permute_in<double>(const int *, double *, const int *, double *,
-                                        const double *, long, int, int, hipStream_t);
+                                        const double *, long, int, int, hipStream_t,
+                                        const double *);
 template int launch_spread<float>(const SpreadArgs<float> &, hipStream_t);
 template int launch_spread<double>(const SpreadArgs<double> &, hipStream_t);
 template int launch_interp<float, float>(const InterpArgs<float, float> &, hipStream_t);

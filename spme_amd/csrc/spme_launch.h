@@ -107,6 +107,9 @@ template <typename T> struct FftArgs {
   // complex, used by the fused plane passes and the z pass; null: the column
   // stays in the spectrum rows (separate passes, grouped applies)
   cpx<T> *nyq;
+  // unequal radii: ncomp = 6 per right-hand side (S_f, S_{a^2 f}) and tab
+  // holds the radius-free factors A(k); see the z pass
+  int poly;
 };
 // True when the single-system apply of this mesh uses the fused x + y plane
 // passes (and then a separate Nyquist column).
@@ -137,7 +140,11 @@ void launch_pgather(const int *perm, const int *base_tmp, const double *w_tmp, i
                     T *pw, int N, int P3, hipStream_t st);
 template <typename T>
 void launch_permute_in(const int *ia, T *fa, const int *ib, T *fb, const T *in, long ldin, int N,
-                       int C, hipStream_t st);
+                       int C, hipStream_t st, const double *rad = nullptr);
+// Unequal radii: out_r = beta out_r + alpha (g[2r] - a_i^2 g[2r+1]).
+template <typename T, typename TO>
+void launch_combine_radii(const T *g, const double *rad, int N, int nrhs, double alpha,
+                          double beta, TO *out, long ldout, hipStream_t st);
 void launch_invert(const int *perm, int *inv, int N, hipStream_t st);
 template <typename T> int launch_spread(const SpreadArgs<T> &a, hipStream_t st);
 template <typename T, typename TO> int launch_interp(const InterpArgs<T, TO> &a, hipStream_t st);
@@ -165,6 +172,6 @@ void launch_influence_table(const double *bsq, int K, int Khp, double dk, double
                             double inv_v, T *tab, hipStream_t st);
 template <typename T>
 void launch_influence_factors(const double *bsq, int K, double dk, double xi, double inv_v,
-                              T *fac, hipStream_t st);
+                              int radius_free, T *fac, hipStream_t st);
 
 }  // namespace spme

@@ -15,6 +15,7 @@ template <typename T, typename TO = T> struct RealArgs {
   const double *spos;  // positions in cell order [N][3]
   double L;
   double self;
+  const T *selfv;  // per-row self coefficients (cell order), or null: self
   const T *fs;  // forces in cell order [N][C]
   int N, C;
   double alpha, beta;
@@ -25,15 +26,17 @@ template <typename T, typename TO = T> struct RealArgs {
 
 void launch_cell_key(const double *pos, int N, double L, int nc, int *key, int *counts,
                      hipStream_t st);
-void launch_cell_gather(const int *perm, const double *pos, double *spos, int N,
-                        hipStream_t st);
+void launch_cell_gather(const int *perm, const double *pos, double *spos, const double *rad,
+                        double *srad, int N, hipStream_t st);
+template <typename T>
+void launch_self_radii(const double *srad, int N, double xi, T *selfv, hipStream_t st);
 void launch_pair_count(const double *spos, const int *coff, int nc, double L, double cut2,
                        int N, int *cnt, hipStream_t st);
 void launch_slice_width(const int *cnt, int N, int *width, int *size, hipStream_t st);
 template <typename T>
 void launch_pair_fill(const double *spos, const int *coff, int nc, double L, double xi,
                       double rc2, double cut2, int N, const int *width, const int *soff,
-                      int *col, T *blk, long E, int *err, hipStream_t st);
+                      int *col, const double *srad, T *blk, long E, int *err, hipStream_t st);
 template <typename T, typename TO> int launch_real(const RealArgs<T, TO> &a, hipStream_t st);
 
 }  // namespace spme

@@ -46,6 +46,66 @@ __device__ __forceinline__ void pair_coeffs(double r2, double xi, double rc2, do
   }
 }
 
+// Unequal radii a_i, a_j (units of the reference radius): the far field is
+// linear in s2 = (a_i^2 + a_j^2)/2, M_far = 3/(4r)(I + rr^T/r^2) + s2/(2r^3)
+// (I - 3 rr^T/r^2), so Beenakker's real-space terms split by dimension into
+// an Oseen part and an s2 part (s2 = 1 gives pair_coeffs):
+//   F = erfc(xi r) (3/(4r) + s2/(2r^3))
+//       + e (3xi^3 r^2 - 9xi/2 + s2 (4xi^7 r^4 - 20xi^5 r^2 + 14xi^3 + xi/r^2))
+//   G = erfc(xi r) (3/(4r) - 3 s2/(2r^3))
+//       + e (-3xi^3 r^2 + 3xi/2 + s2 (-4xi^7 r^4 + 16xi^5 r^2 - 2xi^3 - 3xi/r^2))
+// with e = e^{-xi^2 r^2}/sqrt(pi).  Overlap, r < a_i + a_j (Zuk, Wajnryb,
+// Mizerski, Szymczak 2014): add M_ov - M_far with
+//   |a_i - a_j| < r: M_ov = [16 r^3 (a_i + a_j) - ((a_i - a_j)^2 + 3r^2)^2]
+//                           / (32 r^3 a_i a_j) I
+//                          + 3 ((a_i - a_j)^2 - r^2)^2 / (32 r^3 a_i a_j) rr^T/r^2
+//   r <= |a_i - a_j|: M_ov = I / max(a_i, a_j).
+__device__ __forceinline__ void pair_coeffs_radii(double r2, double xi, double rc2, double ai,
+                                                  double aj, double &F, double &G) {
+  const double r = sqrt(r2), s2 = 0.5 * (ai * ai + aj * aj);
+  F = 0.0;
+  G = 0.0;
+  if (r2 < rc2) {
+    const double xr = xi * r;
+    const double e = exp(-xr * xr) * 0.56418958354775628695;  // 1/sqrt(pi)
+    const double ec = erfc(xr);
+    const double x2 = xi * xi, x3 = x2 * xi, x5 = x3 * x2, x7 = x5 * x2;
+    const double ir = 1.0 / r, ir2 = ir * ir, ir3 = ir2 * ir;
+    F = ec * (0.75 * ir + 0.5 * s2 * ir3) +
+        e * (3.0 * x3 * r2 - 4.5 * xi +
+             s2 * (4.0 * x7 * r2 * r2 - 20.0 * x5 * r2 + 14.0 * x3 + xi * ir2));
+    G = ec * (0.75 * ir - 1.5 * s2 * ir3) +
+        e * (-3.0 * x3 * r2 + 1.5 * xi +
+             s2 * (-4.0 * x7 * r2 * r2 + 16.0 * x5 * r2 - 2.0 * x3 - 3.0 * xi * ir2));
+  }
+  const double sa = ai + aj;
+  if (r < sa) {
+    const double d = ai - aj, r3 = r2 * r;
+    double Fov, Gov;
+    if (r <= fabs(d)) {
+      Fov = 1.0 / fmax(ai, aj);
+      Gov = 0.0;
+    } else {
+      const double q = d * d + 3.0 * r2, w = d * d - r2;
+      Fov = (16.0 * r3 * sa - q * q) / (32.0 * r3 * ai * aj);
+      Gov = 3.0 * w * w / (32.0 * r3 * ai * aj);
+    }
+    F += Fov - (0.75 / r + s2 / (2.0 * r3));
+    G += Gov - (0.75 / r - 1.5 * s2 / r3);
+  }
+}
+
+// Self coefficient of each row (cell order) with unequal radii:
+// 1/a - 6 xi/sqrt(pi) + 40 xi^3 a^2/(3 sqrt(pi)).
+template <typename T>
+__global__ __launch_bounds__(256) void k_self_radii(const double *__restrict__ rrad, int N,
+                                                    double xi, T *__restrict__ selfv) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const double a = rrad[s], isp = 0.56418958354775628695;
+  selfv[s] = T(1.0 / a - 6.0 * xi * isp + 40.0 * xi * xi * xi * a * a * isp / 3.0);
+}
+
 // Cell key of a position (cells of side L/nc, wrapped).
 __global__ __launch_bounds__(256) void k_cell_key(const double *__restrict__ pos, int N,
                                                   double L, int nc, int *__restrict__ key,
@@ -65,15 +125,19 @@ __global__ __launch_bounds__(256) void k_cell_key(const double *__restrict__ pos
   atomicAdd(&counts[k], 1);
 }
 
+// positions (and radii, when rad != null) into cell order
 __global__ __launch_bounds__(256) void k_cell_gather(const int *__restrict__ perm,
                                                      const double *__restrict__ pos,
-                                                     double *__restrict__ spos, int N) {
+                                                     double *__restrict__ spos,
+                                                     const double *__restrict__ rad,
+                                                     double *__restrict__ srad, int N) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   const int i = perm[s];
   spos[3 * s + 0] = pos[3 * i + 0];
   spos[3 * s + 1] = pos[3 * i + 1];
   spos[3 * s + 2] = pos[3 * i + 2];
+  if (rad) srad[s] = rad[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -180,13 +244,15 @@ __global__ __launch_bounds__(256) void k_pair_cols(const double *__restrict__ sp
   for (; k < w; ++k) col[base + 64L * k] = s;
 }
 
-// F and G/r^2 of every stored entry (zero for the padding on the row itself).
+// F and G/r^2 of every stored entry (zero for the padding on the row itself);
+// with srad (radii in cell order) those of the pair's radii.
 template <typename T>
 __global__ __launch_bounds__(256) void k_pair_coeffs(const double *__restrict__ spos, int N,
                                                      double L, double xi, double rc2,
                                                      const int *__restrict__ width,
                                                      const int *__restrict__ soff,
                                                      const int *__restrict__ col,
+                                                     const double *__restrict__ srad,
                                                      T *__restrict__ blk, long E) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
@@ -205,7 +271,10 @@ __global__ __launch_bounds__(256) void k_pair_coeffs(const double *__restrict__ 
       ey -= L * rint(ey * iL);
       ez -= L * rint(ez * iL);
       r2 = ex * ex + ey * ey + ez * ez;
-      pair_coeffs(r2, xi, rc2, F, G);
+      if (srad)
+        pair_coeffs_radii(r2, xi, rc2, srad[s], srad[j], F, G);
+      else
+        pair_coeffs(r2, xi, rc2, F, G);
     }
     blk[e] = T(F);
     blk[E + e] = T(G / r2);
@@ -220,15 +289,16 @@ __global__ __launch_bounds__(256) void k_real_spmv(const int *__restrict__ perm,
                                                    const int *__restrict__ col,
                                                    const T *__restrict__ blk, long E,
                                                    const double *__restrict__ spos, double L,
-                                                   T self, const T *__restrict__ fs, int N,
+                                                   T self, const T *__restrict__ selfv,
+                                                   const T *__restrict__ fs, int N,
                                                    TO alpha, TO beta, int accum,
                                                    TO *__restrict__ out, long ldout) {
   constexpr int LPR = RealShape<C>::LPR;
   const long g = (long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int s = int(g / LPR), l = int(g % LPR);
   if (s < N)
-    real_lane<T, C, TO>(perm, width, soff, col, blk, E, spos, L, self, fs, alpha, beta, accum,
-                        out, ldout, s, l);
+    real_lane<T, C, TO>(perm, width, soff, col, blk, E, spos, L, self, selfv, fs, alpha, beta,
+                        accum, out, ldout, s, l);
 }
 
 static int grid1(long n, int b) { return int((n + b - 1) / b); }
@@ -239,10 +309,18 @@ void launch_cell_key(const double *pos, int N, double L, int nc, int *key, int *
                      counts);
 }
 
-void launch_cell_gather(const int *perm, const double *pos, double *spos, int N,
-                        hipStream_t st) {
-  hipLaunchKernelGGL(k_cell_gather, dim3(grid1(N, 256)), dim3(256), 0, st, perm, pos, spos, N);
+void launch_cell_gather(const int *perm, const double *pos, double *spos, const double *rad,
+                        double *srad, int N, hipStream_t st) {
+  hipLaunchKernelGGL(k_cell_gather, dim3(grid1(N, 256)), dim3(256), 0, st, perm, pos, spos, rad,
+                     srad, N);
 }
+
+template <typename T>
+void launch_self_radii(const double *srad, int N, double xi, T *selfv, hipStream_t st) {
+  hipLaunchKernelGGL(k_self_radii<T>, dim3(grid1(N, 256)), dim3(256), 0, st, srad, N, xi, selfv);
+}
+template void launch_self_radii<float>(const double *, int, double, float *, hipStream_t);
+template void launch_self_radii<double>(const double *, int, double, double *, hipStream_t);
 
 void launch_pair_count(const double *spos, const int *coff, int nc, double L, double cut2,
                        int N, int *cnt, hipStream_t st) {
@@ -257,18 +335,19 @@ void launch_slice_width(const int *cnt, int N, int *width, int *size, hipStream_
 template <typename T>
 void launch_pair_fill(const double *spos, const int *coff, int nc, double L, double xi,
                       double rc2, double cut2, int N, const int *width, const int *soff,
-                      int *col, T *blk, long E, int *err, hipStream_t st) {
+                      int *col, const double *srad, T *blk, long E, int *err, hipStream_t st) {
   hipLaunchKernelGGL(k_pair_cols, dim3(grid1(N, 256)), dim3(256), 0, st, spos, coff, nc, L,
                      cut2, N, width, soff, col, err);
   hipLaunchKernelGGL(k_pair_coeffs<T>, dim3(grid1(N, 256)), dim3(256), 0, st, spos, N, L, xi,
-                     rc2, width, soff, col, blk, E);
+                     rc2, width, soff, col, srad, blk, E);
 }
 
 template <typename T, int C, typename TO>
 static void real_c(const RealArgs<T, TO> &a, hipStream_t st) {
   hipLaunchKernelGGL((k_real_spmv<T, C, TO>), dim3(grid1((long)a.N * RealShape<C>::LPR, 256)),
                      dim3(256), 0, st, a.perm, a.width, a.soff, a.col, a.blk, a.E, a.spos, a.L,
-                     T(a.self), a.fs, a.N, TO(a.alpha), TO(a.beta), a.accum, a.out, a.ldout);
+                     T(a.self), a.selfv, a.fs, a.N, TO(a.alpha), TO(a.beta), a.accum, a.out,
+                     a.ldout);
 }
 
 template <typename T, typename TO>
@@ -284,11 +363,11 @@ int launch_real(const RealArgs<T, TO> &a, hipStream_t st) {
 }
 
 template void launch_pair_fill<float>(const double *, const int *, int, double, double, double,
-                                      double, int, const int *, const int *, int *, float *,
-                                      long, int *, hipStream_t);
+                                      double, int, const int *, const int *, int *,
+                                      const double *, float *, long, int *, hipStream_t);
 template void launch_pair_fill<double>(const double *, const int *, int, double, double, double,
-                                       double, int, const int *, const int *, int *, double *,
-                                       long, int *, hipStream_t);
+                                       double, int, const int *, const int *, int *,
+                                       const double *, double *, long, int *, hipStream_t);
 template int launch_real<float, float>(const RealArgs<float, float> &, hipStream_t);
 template int launch_real<float, double>(const RealArgs<float, double> &, hipStream_t);
 template int launch_real<double, double>(const RealArgs<double, double> &, hipStream_t);

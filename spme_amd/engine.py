@@ -44,11 +44,11 @@ class Engine:
     """One SPME engine (positions, P data, mesh) on the current GPU."""
 
     def __init__(self, xi, dim, porder, np_, box_size, rc=0.0,
-                 precision=SPME_FP64, max_rhs=1):
+                 precision=SPME_FP64, max_rhs=1, radii=False):
         self._lib = load()
         prm = SpmeParams(float(xi), int(dim), int(porder), int(np_),
                          float(box_size), float(rc), int(precision),
-                         int(max_rhs))
+                         int(max_rhs), int(bool(radii)))
         h = ctypes.c_void_p()
         check(self._lib.spme_engine_create(ctypes.byref(prm), ctypes.byref(h)),
               "spme_engine_create")
@@ -106,6 +106,20 @@ class Engine:
         check(self._lib.spme_engine_set_positions(
             self._h, ctypes.c_void_p(pos.data_ptr()), 1, ctypes.c_void_p(s)),
             "set_positions")
+
+    def set_radii(self, radii, stream=None):
+        """Per-particle radii [N] (float64 numpy array or CUDA tensor) of an
+        engine created with radii=True; positions must be set afterwards."""
+        if isinstance(radii, np.ndarray):
+            a = np.ascontiguousarray(radii, dtype=np.float64)
+            check(self._lib.spme_engine_set_radii(
+                self._h, a.ctypes.data_as(ctypes.c_void_p), 0, None), "set_radii")
+            return
+        torch = _torch()
+        assert radii.dtype == torch.float64 and radii.is_cuda and radii.is_contiguous()
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(self._lib.spme_engine_set_radii(
+            self._h, ctypes.c_void_p(radii.data_ptr()), 1, ctypes.c_void_p(s)), "set_radii")
 
     def apply(self, f, u=None, alpha=1.0, beta=0.0, parts=SPME_RECIP,
               stream=None):

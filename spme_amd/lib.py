@@ -26,7 +26,7 @@ EXPORTED = [
     "spme_group_create", "spme_group_destroy", "spme_group_set_tiles",
     "spme_group_get_tiles", "spme_group_autotune", "spme_group_apply",
     "spme_block_gram_worksize", "spme_block_gram", "spme_block_update",
-    "spme_block_gram_noalloc_max", "spme_block_gram_noalloc",
+    "spme_block_gram_noalloc_max", "spme_block_gram_noalloc", "spme_engine_set_radii",
 ]
 
 TILE_FIELDS = ["spread_cc", "spread_zc", "interp_zc", "spread_persist",
@@ -40,7 +40,8 @@ class SpmeParams(ctypes.Structure):
     _fields_ = [("xi", ctypes.c_double), ("dim", ctypes.c_int),
                 ("porder", ctypes.c_int), ("np", ctypes.c_int),
                 ("box_size", ctypes.c_double), ("rc", ctypes.c_double),
-                ("precision", ctypes.c_int), ("max_rhs", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("max_rhs", ctypes.c_int),
+                ("radii", ctypes.c_int)]
 
 
 class SpmeInfo(ctypes.Structure):
@@ -92,6 +93,7 @@ def load() -> ctypes.CDLL:
     L.spme_engine_create.argtypes = [ctypes.POINTER(SpmeParams), ctypes.POINTER(vp)]
     L.spme_engine_destroy.argtypes = [vp]
     L.spme_engine_set_positions.argtypes = [vp, vp, i, vp]
+    L.spme_engine_set_radii.argtypes = [vp, vp, i, vp]
     L.spme_engine_apply.argtypes = [vp, i, d, vp, l, d, vp, l, i, vp]
     L.spme_engine_info.argtypes = [vp, ctypes.POINTER(SpmeInfo)]
     L.spme_engine_debug_stage.argtypes = [vp, i, i, vp, l, vp]

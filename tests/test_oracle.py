@@ -388,3 +388,72 @@ def test_c1_fixture(oracle):
     d = oracle.ewald_direct(s.pos, s.L, f[0], rows=sub)
     ref = g["direct"][0].reshape(-1, 3)[sub].reshape(-1)
     assert np.abs(d - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+# --- unequal radii ------------------------------------------------------------
+
+def test_radii_real_space_pieces(oracle):
+    """The Oseen and s2 parts sum to the unit-radius coefficients; the
+    unequal-sphere overlap correction reduces to Rotne-Prager's for equal
+    radii, vanishes continuously at r = a_i + a_j and is continuous at the
+    containment radius r = |a_i - a_j|."""
+    for r in (0.3, 1.5, 3.0, 6.0):
+        FJ, GJ, Fs, Gs = oracle.real_coeffs_split(r, 0.7)
+        F, G = oracle.real_coeffs(r, 0.7)
+        assert abs(FJ + Fs - F) < 1e-13 * max(1, abs(F)) and abs(GJ + Gs - G) < 1e-13 * max(1, abs(G))
+    for r in (0.5, 1.2, 1.9):
+        assert np.allclose(oracle.overlap_coeffs_radii(r, 1.0, 1.0), oracle.overlap_coeffs(r),
+                           rtol=0, atol=1e-14)
+    ai, aj = 1.7, 0.6
+    assert np.abs(oracle.overlap_coeffs_radii(ai + aj - 1e-10, ai, aj)).max() < 1e-9
+    lo = oracle.overlap_coeffs_radii(ai - aj - 1e-10, ai, aj)
+    hi = oracle.overlap_coeffs_radii(ai - aj + 1e-10, ai, aj)
+    assert np.abs(np.subtract(lo, hi)).max() < 1e-8
+    # inside the larger sphere the pair moves as the larger sphere alone:
+    # M_ov = I / a_max, so F + far field = 1 / a_max, G + far field = 0
+    r = 0.5
+    F, G = oracle.overlap_coeffs_radii(r, ai, aj)
+    s2 = 0.5 * (ai * ai + aj * aj)
+    assert abs(F + 0.75 / r + s2 / (2 * r ** 3) - 1 / ai) < 1e-13
+    assert abs(G + 0.75 / r - 1.5 * s2 / r ** 3) < 1e-12
+
+
+def test_radii_direct_sum(oracle):
+    """Direct Ewald sum with radii: independent of the splitting parameter,
+    equal to the unit-radius sum for unit radii, and Hasimoto's periodic
+    self-mobility 1/a - 2.8372974794/L + 4 pi a^2/(3 L^3) for one sphere of
+    radius a."""
+    N, L = 60, 14.0
+    pos = W.positions(5, N, L)
+    f = W.forces(5, N, 1)[0]
+    d1 = oracle.ewald_direct(pos, L, f, xi=0.5)
+    d1r = oracle.ewald_direct_radii(pos, np.ones(N), L, f, xi=0.5)
+    assert np.abs(d1 - d1r).max() <= 1e-13 * np.abs(d1).max()
+    rad = 0.5 + 1.5 * W.uniform(6, 0, N)
+    a = oracle.ewald_direct_radii(pos, rad, L, f, xi=0.45)
+    b = oracle.ewald_direct_radii(pos, rad, L, f, xi=0.75)
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+    for aa in (0.5, 2.0):
+        u = oracle.ewald_direct_radii(np.array([[3.0, 4.0, 5.0]]), np.array([aa]), 25.0,
+                                      np.array([1.0, 0.0, 0.0]), xi=0.4)
+        ref = 1 / aa - 2.8372974794 / 25.0 + 4 * np.pi * aa * aa / (3 * 25.0 ** 3)
+        assert abs(u[0] - ref) < 1e-10 and abs(u[1]) < 1e-12 and abs(u[2]) < 1e-12
+
+
+def test_radii_spme(oracle):
+    """SPME with radii: unit radii give the unit-radius SPME; the operator is
+    symmetric (the a_i^2 terms of interpolation and spreading are each other's
+    transposes); it converges to the direct sum."""
+    N, L = 60, 14.0
+    pos = W.positions(5, N, L)
+    F = W.forces(7, N, 2)
+    s1 = oracle.spme_full(0.8, 32, 6, 5.0, L, pos, F)
+    s1r = oracle.spme_full_radii(0.8, 32, 6, 5.0, L, pos, np.ones(N), F)
+    assert np.abs(s1 - s1r).max() <= 1e-13 * np.abs(s1).max()
+    rad = 0.5 + 1.5 * W.uniform(6, 0, N)
+    u = oracle.spme_full_radii(0.7, 48, 8, 6.9, L, pos, rad, F)
+    asym = abs(F[0] @ u[1] - F[1] @ u[0]) / (np.linalg.norm(F[0]) * np.linalg.norm(u[1]))
+    assert asym < 1e-13, asym
+    d = oracle.ewald_direct_radii(pos, rad, L, F[0], xi=0.5)
+    err = np.linalg.norm(u[0] - d) / np.linalg.norm(d)
+    assert err < 2e-5, err
