@@ -165,3 +165,46 @@ def test_failed_set_positions_leaves_no_positions(E, torch):
     eng.set_positions(torch.tensor(good, device="cuda"))
     np.testing.assert_array_equal(eng.apply(f, parts=E.SPME_FULL).cpu().numpy(), u_good)
     eng.close()
+
+
+def test_close_overlapping_pairs(oracle, E, torch):
+    """Overlapping pairs at r = 0.01 .. 0.05.  There the real-space and
+    overlap coefficients each hold terms of size 1/(2r^3) (erfc(xi r)/(2r^3)
+    and the far-field subtraction's 1/(2r^3); G three times that) that cancel
+    to O(1): one rounding of erfc or exp, in the engine or in the oracle,
+    moves F and G by about eps/(2r^3).  The FP64 bar of a row is therefore
+    1e-12 of the output scale plus 64 eps |f|_max / (2 r^3) for its closest
+    overlapping pair (2e-9 at r = 0.01, 1e-11 at r = 0.05); a wrong or missing
+    1/r^3 term would be off by about 1/(2 r^3) |f| (5e5 at r = 0.01).  FP32:
+    the coefficients are formed in FP64 after the cancellation, so the usual
+    2e-5 holds."""
+    eps = np.finfo(np.float64).eps
+    N0, L, K, p, xi, rc = 400, W.box_length(400, 0.1), 32, 6, 0.5, 5.0
+    base = W.positions(41, N0, L)
+    rng = np.random.default_rng(41)
+    seps = np.array([0.01, 0.02, 0.03, 0.05])
+    extra = []
+    for r in seps:
+        c = rng.uniform(0, L, 3)
+        d = rng.normal(size=3)
+        d *= r / np.linalg.norm(d)
+        extra += [c, (c + d) % L]
+    pos = np.concatenate([base, np.array(extra)])
+    N = pos.shape[0]
+    f = W.forces(42, N, 2).astype(np.float32).astype(np.float64)
+    rmin = np.full(N, np.inf)
+    for k, r in enumerate(seps):
+        rmin[N0 + 2 * k] = rmin[N0 + 2 * k + 1] = r
+    _say(f"case: {N} particles with overlapping pairs at r = {seps.tolist()}, full M.f")
+    ref = oracle.spme_full(xi, K, p, rc, L, pos, f)
+    got = _apply(E, torch, xi, K, p, L, rc, pos, f, E.SPME_FP64, E.SPME_FULL)
+    scale = np.abs(ref).max()
+    fmax = np.abs(f).max()
+    bound = 1e-12 * scale + np.where(np.isfinite(rmin), 64 * eps * fmax / (2 * rmin ** 3), 0.0)
+    diff = np.abs(got - ref).reshape(2, N, 3).max(axis=(0, 2))
+    worst = float((diff / bound).max())
+    _say(f"  fp64: worst row at {worst:.2e} of its bar; close rows differ by "
+         f"{diff[N0:].max():.2e}, other rows by {diff[:N0].max() / scale:.2e} relative")
+    assert worst <= 1.0
+    got32 = _apply(E, torch, xi, K, p, L, rc, pos, f, E.SPME_FP32, E.SPME_FULL)
+    assert _rel(got32, ref) < FP32_TOL
