@@ -39,6 +39,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's OpenMP threads: one per core, packed (read when the
+# OpenMP runtime starts, i.e. when the CPU leg first loads its library)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 TOL = 1e-4             # target relative error of the batched form
@@ -465,7 +469,9 @@ def main():
                "kind": "port",
                "sample": f"{nr} right-hand side(s) of the same N=200k, K=192, p=8, "
                          f"rc={rc:.3f} FP64 full M.f by the restated reference algorithm "
-                         f"(P build included), {tc:.1f} s"}
+                         f"(P build included; the reference's coloured OpenMP spread with an "
+                         f"even block count, tabulated FFT twiddles, OMP_PLACES="
+                         f"{os.environ.get('OMP_PLACES')}), {tc:.1f} s"}
         print(f"[bench] cpu: {tc:.2f}s for {nr} M.f on {O.num_threads()} threads",
               file=sys.stderr, flush=True)
         # p = 8 is beyond the reference (old-version/spme.c:86 asserts p in

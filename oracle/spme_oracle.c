@@ -215,25 +215,65 @@ void orc_compute_P(int K, int p, const double *pos, int N, double L,
     }
 }
 
-/* spme_kernel.c:231-312 spread, done serially in particle order (the
- * reference's 8-colour parallel form drops blocks when ceil(K/p) is odd).
+/* spme_kernel.c:231-312 spread, in the reference's coloured parallel form:
+ * the mesh is cut into nb^3 blocks of side bs >= p along each axis, every
+ * particle belongs to the block of its stencil's first point, and blocks of
+ * one of the 8 parity colours are spread in parallel (a stencil spans at
+ * most two blocks per axis, so same-colour blocks never write the same
+ * point).  The reference takes nb = ceil(K/p), which races and drops blocks
+ * when nb is odd (spme_kernel.c:268,280); here nb is even (2 floor(K/2p)).
+ * Each mesh point receives its additions in a fixed order (colour, block,
+ * particle index), whatever the thread count.
  * grid: 3 components of K^3, zeroed here.  f: AoS 3N.                      */
 void orc_spread(int K, int p, int N, const double *P, const int *ind,
                 const double *f, double *grid)
 {
     size_t K3 = (size_t)K * K * K;
-    int p3 = p * p * p, i, j;
+    int p3 = p * p * p, i, col;
+    int nb = 2 * (K / (2 * p)), bs, nb3, b;
+    int *cnt, *start, *order;
+    if (nb < 2) nb = 2;
+    bs = (K + nb - 1) / nb;
+    nb3 = nb * nb * nb;
     memset(grid, 0, sizeof(double) * 3 * K3);
+    cnt = (int *)calloc((size_t)nb3 + 1, sizeof(int));
+    start = (int *)malloc(sizeof(int) * ((size_t)nb3 + 1));
+    order = (int *)malloc(sizeof(int) * (size_t)(N > 0 ? N : 1));
+    /* block of each particle from its stencil's first point (x, y, z = 0) */
     for (i = 0; i < N; i++) {
-        for (j = 0; j < p3; j++) {
-            size_t o = (size_t)i * p3 + j;
-            int idx = ind[o];
-            double pv = P[o];
-            grid[0 * K3 + idx] += pv * f[3 * i + 0];
-            grid[1 * K3 + idx] += pv * f[3 * i + 1];
-            grid[2 * K3 + idx] += pv * f[3 * i + 2];
+        int i0 = ind[(size_t)i * p3], xx = i0 % K, yy = (i0 / K) % K, zz = i0 / (K * K);
+        cnt[((zz / bs) * nb + yy / bs) * nb + xx / bs]++;
+    }
+    start[0] = 0;
+    for (b = 0; b < nb3; b++) start[b + 1] = start[b] + cnt[b];
+    for (b = 0; b < nb3; b++) cnt[b] = start[b];
+    for (i = 0; i < N; i++) {
+        int i0 = ind[(size_t)i * p3], xx = i0 % K, yy = (i0 / K) % K, zz = i0 / (K * K);
+        order[cnt[((zz / bs) * nb + yy / bs) * nb + xx / bs]++] = i;
+    }
+    for (col = 0; col < 8; col++) {
+        int h = nb / 2, nbc = h * h * h, t;
+#pragma omp parallel for schedule(dynamic, 1)
+        for (t = 0; t < nbc; t++) {
+            int bx = 2 * (t % h) + (col & 1), by = 2 * ((t / h) % h) + ((col >> 1) & 1);
+            int bz = 2 * (t / (h * h)) + ((col >> 2) & 1);
+            int blk = (bz * nb + by) * nb + bx, q, j;
+            for (q = start[blk]; q < start[blk + 1]; q++) {
+                int ii = order[q];
+                for (j = 0; j < p3; j++) {
+                    size_t o = (size_t)ii * p3 + j;
+                    int idx = ind[o];
+                    double pv = P[o];
+                    grid[0 * K3 + idx] += pv * f[3 * ii + 0];
+                    grid[1 * K3 + idx] += pv * f[3 * ii + 1];
+                    grid[2 * K3 + idx] += pv * f[3 * ii + 2];
+                }
+            }
         }
     }
+    free(cnt);
+    free(start);
+    free(order);
 }
 
 /* spme_kernel.c:8-70 interpolate: u = beta*u + alpha*P^T grid. */
@@ -274,24 +314,37 @@ static int smallest_factor(int n)
     return n;
 }
 
-static void fft_rec(int n, int stride, const double *in, double *out,
-                    int sign, double *tmp)
+/* w[2t], w[2t+1] = cos, sin(2 pi t / n0), t < n0: the twiddles of every
+ * sub-transform (length n divides n0: W_n^e = w[e n0/n]).                 */
+static double *twiddles(int n0)
 {
-    int r, m, q, s, k;
+    double *w = (double *)malloc(sizeof(double) * 2 * (size_t)n0);
+    int t;
+    for (t = 0; t < n0; t++) {
+        w[2 * t] = cos(2.0 * M_PI * t / n0);
+        w[2 * t + 1] = sin(2.0 * M_PI * t / n0);
+    }
+    return w;
+}
+
+static void fft_rec(int n, int stride, const double *in, double *out,
+                    int sign, double *tmp, const double *w, int n0)
+{
+    int r, m, q, s, k, ws;
     if (n == 1) { out[0] = in[0]; out[1] = in[1]; return; }
     r = smallest_factor(n);
     m = n / r;
+    ws = n0 / n;
     /* sub-transforms of the r decimated sequences into out[q*m .. ] */
     for (q = 0; q < r; q++)
-        fft_rec(m, stride * r, in + 2 * q * stride, out + 2 * q * m, sign, tmp);
+        fft_rec(m, stride * r, in + 2 * q * stride, out + 2 * q * m, sign, tmp, w, n0);
     /* butterflies: X[k + s*m] = sum_q W_n^{q(k+s m)} Y_q[k] */
     for (k = 0; k < m; k++) {
         for (s = 0; s < r; s++) {
             double sr = 0.0, si = 0.0;
             for (q = 0; q < r; q++) {
                 long e = (long)q * (k + s * m) % n;
-                double ang = sign * 2.0 * M_PI * (double)e / (double)n;
-                double wr = cos(ang), wi = sin(ang);
+                double wr = w[2 * e * ws], wi = sign * w[2 * e * ws + 1];
                 double yr = out[2 * (q * m + k)], yi = out[2 * (q * m + k) + 1];
                 sr += wr * yr - wi * yi;
                 si += wr * yi + wi * yr;
@@ -303,70 +356,90 @@ static void fft_rec(int n, int stride, const double *in, double *out,
     memcpy(out, tmp, sizeof(double) * 2 * n);
 }
 
+/* 1-D transform of n points in place with a twiddle table of n points and
+ * scratch a, t (2n doubles each). */
+static void fft_line(int n, double *data, int sign, const double *w, double *a, double *t)
+{
+    fft_rec(n, 1, data, a, sign, t, w, n);
+    memcpy(data, a, sizeof(double) * 2 * n);
+}
+
 /* 1-D complex FFT of length n, in place on data[2n]. */
 void orc_fft1d(int n, double *data, int sign)
 {
     double *a = (double *)malloc(sizeof(double) * 2 * n);
     double *t = (double *)malloc(sizeof(double) * 2 * n);
-    fft_rec(n, 1, data, a, sign, t);
-    memcpy(data, a, sizeof(double) * 2 * n);
+    double *w = twiddles(n);
+    fft_line(n, data, sign, w, a, t);
     free(a);
     free(t);
+    free(w);
 }
 
-/* Along y then z of a half spectrum c[(z*K+y)*Kh + x]. */
+/* Along y then z of a half spectrum c[(z*K+y)*Kh + x]; one twiddle table,
+ * scratch per thread. */
 static void fft_yz(int K, double *c, int sign)
 {
-    int Kh = K / 2 + 1, t;
-#pragma omp parallel for schedule(static)
-    for (t = 0; t < K * Kh; t++) {       /* y lines: fixed (z, x) */
-        int z = t / Kh, x = t % Kh, y;
-        double *buf = (double *)malloc(sizeof(double) * 2 * K);
-        for (y = 0; y < K; y++) {
-            size_t o = ((size_t)(z * K + y) * Kh + x) * 2;
-            buf[2 * y] = c[o]; buf[2 * y + 1] = c[o + 1];
+    int Kh = K / 2 + 1;
+    double *w = twiddles(K);
+#pragma omp parallel
+    {
+        double *buf = (double *)malloc(sizeof(double) * 6 * K), *a = buf + 2 * K, *tt = buf + 4 * K;
+        int t;
+#pragma omp for schedule(static)
+        for (t = 0; t < K * Kh; t++) {       /* y lines: fixed (z, x) */
+            int z = t / Kh, x = t % Kh, y;
+            for (y = 0; y < K; y++) {
+                size_t o = ((size_t)(z * K + y) * Kh + x) * 2;
+                buf[2 * y] = c[o]; buf[2 * y + 1] = c[o + 1];
+            }
+            fft_line(K, buf, sign, w, a, tt);
+            for (y = 0; y < K; y++) {
+                size_t o = ((size_t)(z * K + y) * Kh + x) * 2;
+                c[o] = buf[2 * y]; c[o + 1] = buf[2 * y + 1];
+            }
         }
-        orc_fft1d(K, buf, sign);
-        for (y = 0; y < K; y++) {
-            size_t o = ((size_t)(z * K + y) * Kh + x) * 2;
-            c[o] = buf[2 * y]; c[o + 1] = buf[2 * y + 1];
+#pragma omp for schedule(static)
+        for (t = 0; t < K * Kh; t++) {       /* z lines: fixed (y, x) */
+            int y = t / Kh, x = t % Kh, z;
+            for (z = 0; z < K; z++) {
+                size_t o = ((size_t)(z * K + y) * Kh + x) * 2;
+                buf[2 * z] = c[o]; buf[2 * z + 1] = c[o + 1];
+            }
+            fft_line(K, buf, sign, w, a, tt);
+            for (z = 0; z < K; z++) {
+                size_t o = ((size_t)(z * K + y) * Kh + x) * 2;
+                c[o] = buf[2 * z]; c[o + 1] = buf[2 * z + 1];
+            }
         }
         free(buf);
     }
-#pragma omp parallel for schedule(static)
-    for (t = 0; t < K * Kh; t++) {       /* z lines: fixed (y, x) */
-        int y = t / Kh, x = t % Kh, z;
-        double *buf = (double *)malloc(sizeof(double) * 2 * K);
-        for (z = 0; z < K; z++) {
-            size_t o = ((size_t)(z * K + y) * Kh + x) * 2;
-            buf[2 * z] = c[o]; buf[2 * z + 1] = c[o + 1];
-        }
-        orc_fft1d(K, buf, sign);
-        for (z = 0; z < K; z++) {
-            size_t o = ((size_t)(z * K + y) * Kh + x) * 2;
-            c[o] = buf[2 * z]; c[o + 1] = buf[2 * z + 1];
-        }
-        free(buf);
-    }
+    free(w);
 }
 
 /* 3-D real-to-complex forward transform, unnormalised.  r: K^3 reals;
  * c: K*K*(K/2+1) complex.  Same transform as spme.c:654-656.               */
 void orc_fft3d_r2c(int K, const double *r, double *c)
 {
-    int Kh = K / 2 + 1, t;
-#pragma omp parallel for schedule(static)
-    for (t = 0; t < K * K; t++) {
-        double *buf = (double *)malloc(sizeof(double) * 2 * K);
-        int x;
-        for (x = 0; x < K; x++) { buf[2 * x] = r[(size_t)t * K + x]; buf[2 * x + 1] = 0.0; }
-        orc_fft1d(K, buf, -1);
-        for (x = 0; x < Kh; x++) {
-            c[((size_t)t * Kh + x) * 2] = buf[2 * x];
-            c[((size_t)t * Kh + x) * 2 + 1] = buf[2 * x + 1];
+    int Kh = K / 2 + 1;
+    double *w = twiddles(K);
+#pragma omp parallel
+    {
+        double *buf = (double *)malloc(sizeof(double) * 6 * K), *a = buf + 2 * K, *tt = buf + 4 * K;
+        int t;
+#pragma omp for schedule(static)
+        for (t = 0; t < K * K; t++) {
+            int x;
+            for (x = 0; x < K; x++) { buf[2 * x] = r[(size_t)t * K + x]; buf[2 * x + 1] = 0.0; }
+            fft_line(K, buf, -1, w, a, tt);
+            for (x = 0; x < Kh; x++) {
+                c[((size_t)t * Kh + x) * 2] = buf[2 * x];
+                c[((size_t)t * Kh + x) * 2 + 1] = buf[2 * x + 1];
+            }
         }
         free(buf);
     }
+    free(w);
     fft_yz(K, c, -1);
 }
 
@@ -378,26 +451,33 @@ void orc_fft3d_r2c(int K, const double *r, double *c)
  * the real part of the inverse DFT of the stored spectrum.                 */
 void orc_fft3d_c2r(int K, double *c, double *r)
 {
-    int Kh = K / 2 + 1, t;
+    int Kh = K / 2 + 1;
+    double *w;
     fft_yz(K, c, +1);
-#pragma omp parallel for schedule(static)
-    for (t = 0; t < K * K; t++) {
-        double *buf = (double *)malloc(sizeof(double) * 2 * K);
-        int x;
-        for (x = 0; x < Kh; x++) {
-            buf[2 * x] = c[((size_t)t * Kh + x) * 2];
-            buf[2 * x + 1] = c[((size_t)t * Kh + x) * 2 + 1];
+    w = twiddles(K);
+#pragma omp parallel
+    {
+        double *buf = (double *)malloc(sizeof(double) * 6 * K), *a = buf + 2 * K, *tt = buf + 4 * K;
+        int t;
+#pragma omp for schedule(static)
+        for (t = 0; t < K * K; t++) {
+            int x;
+            for (x = 0; x < Kh; x++) {
+                buf[2 * x] = c[((size_t)t * Kh + x) * 2];
+                buf[2 * x + 1] = c[((size_t)t * Kh + x) * 2 + 1];
+            }
+            buf[1] = 0.0;
+            buf[2 * (K / 2) + 1] = 0.0;
+            for (x = Kh; x < K; x++) {
+                buf[2 * x] = buf[2 * (K - x)];
+                buf[2 * x + 1] = -buf[2 * (K - x) + 1];
+            }
+            fft_line(K, buf, +1, w, a, tt);
+            for (x = 0; x < K; x++) r[(size_t)t * K + x] = buf[2 * x];
         }
-        buf[1] = 0.0;
-        buf[2 * (K / 2) + 1] = 0.0;
-        for (x = Kh; x < K; x++) {
-            buf[2 * x] = buf[2 * (K - x)];
-            buf[2 * x + 1] = -buf[2 * (K - x) + 1];
-        }
-        orc_fft1d(K, buf, +1);
-        for (x = 0; x < K; x++) r[(size_t)t * K + x] = buf[2 * x];
         free(buf);
     }
+    free(w);
 }
 
 /* spme_kernel.c:114-228 apply_influence on 3 half spectra (re/im pairs):
