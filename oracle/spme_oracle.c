@@ -216,25 +216,32 @@ void orc_compute_P(int K, int p, const double *pos, int N, double L,
 }
 
 /* spme_kernel.c:231-312 spread, in the reference's coloured parallel form:
- * the mesh is cut into nb^3 blocks of side bs >= p along each axis, every
- * particle belongs to the block of its stencil's first point, and blocks of
- * one of the 8 parity colours are spread in parallel (a stencil spans at
- * most two blocks per axis, so same-colour blocks never write the same
- * point).  The reference takes nb = ceil(K/p), which races and drops blocks
- * when nb is odd (spme_kernel.c:268,280); here nb is even (2 floor(K/2p)).
- * Each mesh point receives its additions in a fixed order (colour, block,
- * particle index), whatever the thread count.
+ * each axis is cut into nb blocks [floor(b K/nb), floor((b+1) K/nb)), all at
+ * least p wide, every particle belongs to the block of its stencil's first
+ * point, and the blocks of one of the 8 parity colours are spread in
+ * parallel: a stencil spans at most two consecutive blocks per axis
+ * (periodically: block nb-1 is followed by block 0, of the other parity
+ * since nb is even), so same-colour blocks never write the same point.  The
+ * reference takes nb = ceil(K/p) blocks of side p, which races and drops
+ * blocks when nb is odd (spme_kernel.c:268,280) and leaves a short last
+ * block when p does not divide K; here nb = 2 floor(K/2p) and the blocks
+ * tile the axis exactly.  Each mesh point receives its additions in a fixed
+ * order (colour, block, particle index), whatever the thread count.
  * grid: 3 components of K^3, zeroed here.  f: AoS 3N.                      */
 void orc_spread(int K, int p, int N, const double *P, const int *ind,
                 const double *f, double *grid)
 {
     size_t K3 = (size_t)K * K * K;
     int p3 = p * p * p, i, col;
-    int nb = 2 * (K / (2 * p)), bs, nb3, b;
-    int *cnt, *start, *order;
+    int nb = 2 * (K / (2 * p)), nb3, b;
+    int *cnt, *start, *order, *bo;
     if (nb < 2) nb = 2;
-    bs = (K + nb - 1) / nb;
     nb3 = nb * nb * nb;
+    bo = (int *)malloc(sizeof(int) * K);  /* block of each mesh index */
+    for (b = 0; b < nb; b++) {
+        int x;
+        for (x = (int)((long)b * K / nb); x < (int)((long)(b + 1) * K / nb); x++) bo[x] = b;
+    }
     memset(grid, 0, sizeof(double) * 3 * K3);
     cnt = (int *)calloc((size_t)nb3 + 1, sizeof(int));
     start = (int *)malloc(sizeof(int) * ((size_t)nb3 + 1));
@@ -242,14 +249,14 @@ void orc_spread(int K, int p, int N, const double *P, const int *ind,
     /* block of each particle from its stencil's first point (x, y, z = 0) */
     for (i = 0; i < N; i++) {
         int i0 = ind[(size_t)i * p3], xx = i0 % K, yy = (i0 / K) % K, zz = i0 / (K * K);
-        cnt[((zz / bs) * nb + yy / bs) * nb + xx / bs]++;
+        cnt[(bo[zz] * nb + bo[yy]) * nb + bo[xx]]++;
     }
     start[0] = 0;
     for (b = 0; b < nb3; b++) start[b + 1] = start[b] + cnt[b];
     for (b = 0; b < nb3; b++) cnt[b] = start[b];
     for (i = 0; i < N; i++) {
         int i0 = ind[(size_t)i * p3], xx = i0 % K, yy = (i0 / K) % K, zz = i0 / (K * K);
-        order[cnt[((zz / bs) * nb + yy / bs) * nb + xx / bs]++] = i;
+        order[cnt[(bo[zz] * nb + bo[yy]) * nb + bo[xx]]++] = i;
     }
     for (col = 0; col < 8; col++) {
         int h = nb / 2, nbc = h * h * h, t;
@@ -274,6 +281,7 @@ void orc_spread(int K, int p, int N, const double *P, const int *ind,
     free(cnt);
     free(start);
     free(order);
+    free(bo);
 }
 
 /* spme_kernel.c:8-70 interpolate: u = beta*u + alpha*P^T grid. */

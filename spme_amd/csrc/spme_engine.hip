@@ -168,8 +168,9 @@ struct _spme_t {
   int *r_cnt, *r_width, *r_size, *r_soff, *r_col;
   void *r_blk;
   long r_E, r_cap, r_used;
-  // drop-in staging (FP64 host API)
+  // drop-in staging (FP64 host API), h_rhs right-hand sides of 3N values
   double *h_in, *h_out;
+  int h_rhs;
   // per-kernel timing: when enabled, events are recorded on the apply
   // stream after every launch; a stage's time is the gap to the event before
   int timing;
@@ -513,6 +514,33 @@ int set_radii(_spme_t *e, const double *rad, int on_device, hipStream_t st) {
   e->cut = cut;
   HIP_TRY(hipMemcpyAsync(e->rad, h.data(), sizeof(double) * e->N, hipMemcpyHostToDevice, st));
   HIP_TRY(hipStreamSynchronize(st));
+  return 0;
+}
+
+// Room for batches of up to R right-hand sides: the mesh, Nyquist column,
+// sorted forces and (with radii) interpolation buffers are reallocated; the
+// P data and block list do not depend on the batch width and stay.  Groups
+// holding the engine see a new position version and rebuild their tables.
+int engine_grow(_spme_t *e, int R) {
+  if (R <= e->max_rhs) return 0;
+  if (R > 16 || (e->poly && R > 8)) return fail(-1, "batch of %d right-hand sides too large", R);
+  HIP_TRY(hipStreamSynchronize(e->stream));
+  const size_t C = (size_t)e->Cm * R;
+  void **bufs[] = {&e->grid, &e->nyq, (void **)&e->fs, &e->r_fs, &e->tmp};
+  const bool had[] = {e->grid != nullptr, e->nyq != nullptr, e->fs != nullptr,
+                      e->r_fs != nullptr, e->tmp != nullptr};
+  const size_t bytes[] = {(e->dense ? 3 : C) * e->comp_stride * e->tsz,
+                          C * e->K * e->K * 2 * e->tsz, e->tsz * C * e->N,
+                          e->tsz * (size_t)3 * R * e->N, e->tsz * C * e->N};
+  for (int b = 0; b < 5; ++b) {
+    if (!had[b]) continue;
+    (void)hipFree(*bufs[b]);
+    *bufs[b] = nullptr;
+  }
+  e->max_rhs = e->prm.max_rhs = R;
+  ++e->pos_version;
+  for (int b = 0; b < 5; ++b)
+    if (had[b]) HIP_TRY(hipMalloc(bufs[b], bytes[b]));
   return 0;
 }
 
@@ -1494,6 +1522,7 @@ void create_spme_engine(double xi, int dim, int porder, int np, double box_size,
     fail(-2, "hipMalloc of staging vectors failed");
     die("create_spme_engine");
   }
+  (*_spme)->h_rhs = 1;
 }
 
 void destroy_spme_engine(spme_t *spme) { spme_engine_destroy(spme); }
@@ -1510,22 +1539,48 @@ void compute_spme(spme_t *spme, double *pos, int nrhs, double alpha, double *vec
     fail(-1, "the first call must pass positions");
     die("compute_spme");
   }
+  if (nrhs <= 0) return;
+  if (ldin < 3 * spme->N || ldout < 3 * spme->N) {
+    fail(-1, "ldin/ldout must be >= 3*np");
+    die("compute_spme");
+  }
+  // right-hand sides go through the batched apply (the reference loops
+  // them one by one, old-version/spme.c:631-702; FP64 batches give bitwise
+  // the single results), in batches whose mesh stays within 4 GiB
   const size_t n3 = 3 * (size_t)spme->N;
-  for (int nn = 0; nn < nrhs; ++nn) {
+  const double per_rhs = 3.0 * spme->comp_stride * spme->tsz;
+  const int cap = int(std::max(1.0, std::min(16.0, std::floor(4294967296.0 / per_rhs))));
+  const int R = std::min(nrhs, cap);
+  if (engine_grow(spme, R)) die("compute_spme");
+  if (spme->h_rhs < R) {
+    (void)hipFree(spme->h_in);
+    (void)hipFree(spme->h_out);
+    spme->h_in = spme->h_out = nullptr;
+    spme->h_rhs = 0;
+    if (hipMalloc(&spme->h_in, sizeof(double) * n3 * R) != hipSuccess ||
+        hipMalloc(&spme->h_out, sizeof(double) * n3 * R) != hipSuccess) {
+      fail(-2, "hipMalloc of staging vectors failed");
+      die("compute_spme");
+    }
+    spme->h_rhs = R;
+  }
+  const size_t w = n3 * sizeof(double);
+  for (int nn = 0; nn < nrhs; nn += R) {
+    const int nb = std::min(R, nrhs - nn);
     const double *vin = vec_in + (size_t)nn * ldin;
     double *vout = vec_out + (size_t)nn * ldout;
-    if (hipMemcpyAsync(spme->h_in, vin, n3 * sizeof(double), hipMemcpyHostToDevice, st) !=
-            hipSuccess ||
-        (beta != 0.0 && hipMemcpyAsync(spme->h_out, vout, n3 * sizeof(double),
-                                       hipMemcpyHostToDevice, st) != hipSuccess)) {
+    if (hipMemcpy2DAsync(spme->h_in, w, vin, sizeof(double) * ldin, w, nb,
+                         hipMemcpyHostToDevice, st) != hipSuccess ||
+        (beta != 0.0 && hipMemcpy2DAsync(spme->h_out, w, vout, sizeof(double) * ldout, w, nb,
+                                          hipMemcpyHostToDevice, st) != hipSuccess)) {
       fail(-2, "host-to-device copy failed");
       die("compute_spme");
     }
-    if (apply(spme, 1, alpha, spme->h_in, (long)n3, beta, spme->h_out, (long)n3, SPME_RECIP,
+    if (apply(spme, nb, alpha, spme->h_in, (long)n3, beta, spme->h_out, (long)n3, SPME_RECIP,
               st))
       die("compute_spme");
-    if (hipMemcpyAsync(vout, spme->h_out, n3 * sizeof(double), hipMemcpyDeviceToHost, st) !=
-            hipSuccess ||
+    if (hipMemcpy2DAsync(vout, sizeof(double) * ldout, spme->h_out, w, w, nb,
+                         hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess) {
       fail(-2, "device-to-host copy failed");
       die("compute_spme");

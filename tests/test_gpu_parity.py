@@ -66,6 +66,36 @@ def test_dropin_matches_oracle_c1(oracle, E, torch):
     d.close()
 
 
+def test_dropin_batches_many_rhs(oracle, E, torch):
+    """compute_spme with 20 right-hand sides, padded ldin/ldout and alpha,
+    beta: the drop-in runs them as batched applies (16 + 4 here); the result
+    equals the oracle and, bitwise, one compute_spme call per right-hand
+    side (the reference's loop, old-version/spme.c:631-702)."""
+    _say("case: drop-in compute_spme, 20 RHS in batches, padded strides")
+    N, K, p, R, ld = 600, 32, 6, 20, 1808
+    pos, L = _system(61, N, 0.1)
+    xi = 0.7
+    f = np.zeros((R, ld))
+    f[:, :3 * N] = W.forces(61, N, R)
+    v0 = np.full((R, ld), 5.0)
+    v0[:, :3 * N] = W.forces(62, N, R)
+    ref = oracle.compute_spme(xi, K, p, L, pos, f[:, :3 * N], alpha=2.1, beta=2.2,
+                              u=v0[:, :3 * N])
+    d = E.DropIn(xi, K, p, N, L)
+    v = v0.copy()
+    d.compute(pos, R, 2.1, f, ld, 2.2, v, ld)
+    assert _rel(v[:, :3 * N], ref) < FP64_TOL
+    assert np.all(v[:, 3 * N:] == 5.0)   # padding untouched
+    one = E.DropIn(xi, K, p, N, L)
+    for r in range(R):
+        vr = np.ascontiguousarray(v0[r])
+        one.compute(pos if r == 0 else None, 1, 2.1, np.ascontiguousarray(f[r]), ld, 2.2, vr,
+                    ld)
+        np.testing.assert_array_equal(vr, v[r])
+    d.close()
+    one.close()
+
+
 @pytest.mark.parametrize("p,K", [(4, 32), (6, 32), (8, 48)])
 def test_spread_and_fft_stages(oracle, E, torch, p, K):
     _say(f"case: stages p={p} K={K} N=300")

@@ -457,3 +457,20 @@ def test_radii_spme(oracle):
     d = oracle.ewald_direct_radii(pos, rad, L, F[0], xi=0.5)
     err = np.linalg.norm(u[0] - d) / np.linalg.norm(d)
     assert err < 2e-5, err
+
+
+@pytest.mark.parametrize("K,p", [(128, 6), (48, 8), (24, 6), (96, 8), (64, 6)])
+def test_coloured_spread_equals_serial_sum(oracle, K, p):
+    """The coloured parallel spread (blocks of one parity colour spread
+    concurrently) equals the plain particle-by-particle sum, including
+    meshes that p does not divide (K = 128, p = 6: a stencil that wraps from
+    the last block into block 0) and densely packed particles."""
+    N = 3000
+    pos = W.positions(17, N, 7.0)   # 3000 particles in a box of side 7: many per block
+    f = W.forces(17, N, 1)[0]
+    P, ind = oracle.compute_P(K, p, pos, 7.0)[:2]
+    g = oracle.spread(K, p, P, ind, f)
+    ref = np.zeros((3, K ** 3))
+    for c in range(3):
+        np.add.at(ref[c], ind.reshape(-1), (P * f[c::3][:, None]).reshape(-1))
+    assert np.abs(g.reshape(3, -1) - ref).max() <= 1e-12 * np.abs(ref).max()
