@@ -1080,12 +1080,15 @@ __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
     const T b0 = fma(gx, x, F), b1 = gx * y, b2 = gx * z;
     const T b3 = fma(gy, y, F), b4 = gy * z, b5 = fma(gz, z, F);
     load_n<T, NV>(fs + (size_t)j * C + 3 * r0, f);
+    // each neighbour's product is formed apart from the running sums and
+    // added last, so that the gathers of the unrolled neighbours do not wait
+    // on one another's arithmetic
 #pragma unroll
     for (int r = 0; r < RPL; ++r) {
       const T f0 = f[3 * r], f1 = f[3 * r + 1], f2 = f[3 * r + 2];
-      a[3 * r] = fma(b2, f2, fma(b1, f1, fma(b0, f0, a[3 * r])));
-      a[3 * r + 1] = fma(b4, f2, fma(b3, f1, fma(b1, f0, a[3 * r + 1])));
-      a[3 * r + 2] = fma(b5, f2, fma(b4, f1, fma(b2, f0, a[3 * r + 2])));
+      a[3 * r] += fma(b2, f2, fma(b1, f1, b0 * f0));
+      a[3 * r + 1] += fma(b4, f2, fma(b3, f1, b1 * f0));
+      a[3 * r + 2] += fma(b5, f2, fma(b4, f1, b2 * f0));
     }
   }
   const long po = 3L * perm[s];
