@@ -24,6 +24,18 @@ __device__ __forceinline__ void permute_elem(const int *__restrict__ perm,
   fs[t] = in[r * ldin + 3L * perm[s] + d];
 }
 
+// The same into the real-space layout (cell order, the right-hand sides of
+// one component together): fs[s*C + d*R + r], R = C/3; element t of the
+// N*C outputs.
+template <typename T>
+__device__ __forceinline__ void permute_elem_cell(const int *__restrict__ perm,
+                                                  const T *__restrict__ in, long ldin,
+                                                  T *__restrict__ fs, int C, long t) {
+  const int s = int(t / C), c = int(t - (long)s * C);
+  const int R = C / 3, d = c / R, r = c - d * R;
+  fs[t] = in[r * ldin + 3L * perm[s] + d];
+}
+
 // ---------------------------------------------------------------------------
 // Spread (old-version/spme_kernel.c:231-312): grid[c] = sum_i q_i (x) f_ic.
 // One wave owns a TX x TY column of cells over a z range and keeps a window
@@ -995,16 +1007,23 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
 
 // Real-space SpMV: a row (cell order) is handled by LPR lanes, each with RPL
 // consecutive right-hand sides, so the pair's coefficients and column index
-// are loaded once per RPL right-hand sides and the neighbour's 3 RPL forces
-// come as 16-byte loads.  fs: forces in cell order [N][C].  For right-hand
-// side r and d < 3:
+// are loaded once per RPL right-hand sides.  fs: forces in cell order with
+// the right-hand sides of one component together, fs[s*C + d*R + r] (R =
+// C/3), so that for each d the LPR lanes of a row read one contiguous
+// segment of R values.  For right-hand side r and d < 3:
 //   out[r*ldout + 3*perm[s] + d] = (accum ? out : beta*out)
 //                                  + alpha*(self f + sum_j B_sj f_j),
 // B_sj = F I + (G/r^2) e e^T with e the minimum-image separation, formed
-// from the stored FP64-built F and G/r^2 and the FP64 positions (8 stored
-// bytes per entry instead of 6 block values).  With unequal radii the
-// stored F and G carry the pair's radii and the self coefficient is per row
-// (selfv, cell order; null: the scalar self).
+// from the stored FP64-built pair (F, G/r^2) (interleaved, one 2-value load
+// per entry) and the positions.  FP64 engines take e from the FP64
+// positions.  FP32 engines take it from cpos: per particle its offset from
+// the corner of its cell, in FP32 (at most one cell side, so within about
+// 1e-6 of the FP64 value), and the cell's coordinates; e = offset_j -
+// offset_i + dc*h with dc the wrapped cell difference (-1, 0 or 1: a stored
+// neighbour lies in one of the 27 cells around the row's, and there are at
+// least 3 cells per side).  With unequal radii the stored F and G carry the
+// pair's radii and the self coefficient is per row (selfv, cell order;
+// null: the scalar self).
 // Rows are packed in slices of 64 (entry-major within a slice); the rows of
 // a wave lie in one slice, so every lane runs the same entry count.
 template <int C> struct RealShape {
@@ -1039,56 +1058,145 @@ __device__ __forceinline__ void load_n(const T *__restrict__ p, T (&v)[N]) {
   }
 }
 
+// Cell coordinates packed into cpos[s].w: cx | cy << 10 | cz << 20.
+struct CellPos {
+  float4 v;
+  __device__ __forceinline__ int c(int d) const {
+    return (__float_as_int(v.w) >> (10 * d)) & 1023;
+  }
+};
+
+// Wrapped difference of two cell coordinates of neighbouring cells.
+__device__ __forceinline__ int cell_step(int cj, int ci, int nc) {
+  const int d = cj - ci;
+  return d > 1 ? d - nc : (d < -1 ? d + nc : d);
+}
+
 // TO: output type (see interp_block); the pair sums are formed in T.
 template <typename T, int C, typename TO = T>
 __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
                                           const int *__restrict__ width,
                                           const int *__restrict__ soff,
                                           const int *__restrict__ col,
-                                          const T *__restrict__ blk, long E,
-                                          const double *__restrict__ spos, double L, T self,
-                                          const T *__restrict__ selfv,
+                                          const T *__restrict__ blk,
+                                          const double *__restrict__ spos,
+                                          const float4 *__restrict__ cpos, int nc, float h,
+                                          double L, T self, const T *__restrict__ selfv,
                                           const T *__restrict__ fs, TO alpha, TO beta, int accum,
                                           TO *__restrict__ out, long ldout, int s, int l) {
-  constexpr int RPL = RealShape<C>::RPL, NV = 3 * RPL;
+  constexpr int RPL = RealShape<C>::RPL, R = C / 3;
+  constexpr bool F32 = sizeof(T) == 4;
   const int q = s >> 6, ls = s & 63;
   const int r0 = l * RPL;
-  T a[NV], f[NV];
-  load_n<T, NV>(fs + (size_t)s * C + 3 * r0, f);
+  T a[3][RPL], f[3][RPL];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) load_n<T, RPL>(fs + (size_t)s * C + d * R + r0, f[d]);
   // self term: one coefficient, or one per row (unequal radii)
   const T sf = selfv ? selfv[s] : self;
 #pragma unroll
-  for (int i = 0; i < NV; ++i) a[i] = sf * f[i];
-  const double px = spos[3L * s], py = spos[3L * s + 1], pz = spos[3L * s + 2];
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) a[d][r] = sf * f[d][r];
+  double px = 0.0, py = 0.0, pz = 0.0;
+  CellPos ci{};
+  if constexpr (F32) {
+    ci.v = cpos[s];
+  } else {
+    px = spos[3L * s];
+    py = spos[3L * s + 1];
+    pz = spos[3L * s + 2];
+  }
   const double iL = 1.0 / L;
   const long base = soff[q] + ls;
   const int w = width[q];
-#pragma unroll 4
-  for (int k = 0; k < w; ++k) {
+  struct alignas(2 * sizeof(T)) Pair { T F, g; };
+  const Pair *pairs = reinterpret_cast<const Pair *>(blk) + base;
+  const int *cols = col + base;
+  // U neighbours per step.  The column indices and coefficients of the next
+  // step are loaded while the current step's gathers are in flight, and all
+  // U gathers are issued before any of them is used.  Entries past the row's
+  // width repeat its last entry with zero coefficients (they add exact
+  // zeros), so no load leaves the slice.
+  constexpr int U = 4;
+  int jn[U];
+  Pair gn[U];
+  auto fetch_entries = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long k = min(k0 + u, w - 1);
+      jn[u] = cols[64L * k];
+      gn[u] = pairs[64L * k];
+    }
+  };
+  if (w > 0) fetch_entries(0);
+  for (int k0 = 0; k0 < w; k0 += U) {
     // every fused multiply-add is written out, so that the instantiations
     // for different batch widths (right-hand sides per lane) round alike
 #pragma clang fp contract(off)
-    const long e = base + 64L * k;
-    const int j = col[e];
-    const T F = blk[e], g = blk[E + e];
-    double ex = spos[3L * j] - px, ey = spos[3L * j + 1] - py, ez = spos[3L * j + 2] - pz;
-    ex = fma(-L, rint(ex * iL), ex);
-    ey = fma(-L, rint(ey * iL), ey);
-    ez = fma(-L, rint(ez * iL), ez);
-    const T x = T(ex), y = T(ey), z = T(ez);
-    const T gx = g * x, gy = g * y, gz = g * z;
-    const T b0 = fma(gx, x, F), b1 = gx * y, b2 = gx * z;
-    const T b3 = fma(gy, y, F), b4 = gy * z, b5 = fma(gz, z, F);
-    load_n<T, NV>(fs + (size_t)j * C + 3 * r0, f);
-    // each neighbour's product is formed apart from the running sums and
-    // added last, so that the gathers of the unrolled neighbours do not wait
-    // on one another's arithmetic
+    int j[U];
+    T F[U], g[U];
 #pragma unroll
-    for (int r = 0; r < RPL; ++r) {
-      const T f0 = f[3 * r], f1 = f[3 * r + 1], f2 = f[3 * r + 2];
-      a[3 * r] += fma(b2, f2, fma(b1, f1, b0 * f0));
-      a[3 * r + 1] += fma(b4, f2, fma(b3, f1, b1 * f0));
-      a[3 * r + 2] += fma(b5, f2, fma(b4, f1, b2 * f0));
+    for (int u = 0; u < U; ++u) {
+      j[u] = jn[u];
+      const bool on = k0 + u < w;
+      F[u] = on ? gn[u].F : T(0);
+      g[u] = on ? gn[u].g : T(0);
+    }
+    T fv[U][3][RPL];
+    T x[U], y[U], z[U];
+    if constexpr (F32) {
+      CellPos cj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) cj[u].v = cpos[j[u]];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) load_n<T, RPL>(fs + (size_t)j[u] * C + d * R + r0, fv[u][d]);
+      if (k0 + U < w) fetch_entries(k0 + U);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        x[u] = (cj[u].v.x - ci.v.x) + float(cell_step(cj[u].c(0), ci.c(0), nc)) * h;
+        y[u] = (cj[u].v.y - ci.v.y) + float(cell_step(cj[u].c(1), ci.c(1), nc)) * h;
+        z[u] = (cj[u].v.z - ci.v.z) + float(cell_step(cj[u].c(2), ci.c(2), nc)) * h;
+      }
+    } else {
+      double ex[U], ey[U], ez[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        ex[u] = spos[3L * j[u]];
+        ey[u] = spos[3L * j[u] + 1];
+        ez[u] = spos[3L * j[u] + 2];
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) load_n<T, RPL>(fs + (size_t)j[u] * C + d * R + r0, fv[u][d]);
+      if (k0 + U < w) fetch_entries(k0 + U);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        double dx = ex[u] - px, dy = ey[u] - py, dz = ez[u] - pz;
+        dx = fma(-L, rint(dx * iL), dx);
+        dy = fma(-L, rint(dy * iL), dy);
+        dz = fma(-L, rint(dz * iL), dz);
+        x[u] = T(dx);
+        y[u] = T(dy);
+        z[u] = T(dz);
+      }
+    }
+    // each neighbour's product is formed apart from the running sums and
+    // added last, in entry order
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T gx = g[u] * x[u], gy = g[u] * y[u], gz = g[u] * z[u];
+      const T b0 = fma(gx, x[u], F[u]), b1 = gx * y[u], b2 = gx * z[u];
+      const T b3 = fma(gy, y[u], F[u]), b4 = gy * z[u], b5 = fma(gz, z[u], F[u]);
+#pragma unroll
+      for (int r = 0; r < RPL; ++r) {
+        const T f0 = fv[u][0][r], f1 = fv[u][1][r], f2 = fv[u][2][r];
+        a[0][r] += fma(b2, f2, fma(b1, f1, b0 * f0));
+        a[1][r] += fma(b4, f2, fma(b3, f1, b1 * f0));
+        a[2][r] += fma(b5, f2, fma(b4, f1, b2 * f0));
+      }
     }
   }
   const long po = 3L * perm[s];
@@ -1099,7 +1207,7 @@ __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
     for (int d = 0; d < 3; ++d) {
 #pragma clang fp contract(off)
       const TO prev = accum ? u[d] : (beta == TO(0) ? TO(0) : beta * u[d]);
-      u[d] = fma(alpha, TO(a[3 * r + d]), prev);
+      u[d] = fma(alpha, TO(a[d][r]), prev);
     }
   }
 }

@@ -162,6 +162,7 @@ struct _spme_t {
   int *r_key, *r_counts, *r_cursor, *r_off, *r_perm;
   int *iperm, *r_iperm;  // inverses of perm and r_perm (original -> sorted)
   double *r_pos;
+  float4 *r_cpos;  // FP32 engines: cell-relative positions for the SpMV
   void *r_fs;
   // real-space pair blocks (sliced ELL, 64 rows per slice)
   int nslices;
@@ -267,7 +268,7 @@ void free_all(_spme_t *e) {
   void *ptrs[] = {e->grid, e->nyq, e->tab, e->tw, e->pos, e->base_tmp, e->key_tmp, e->counts,
                   e->cursor, e->boff, e->perm, e->pbase, e->err, e->w_tmp, e->pw, e->fs,
                   e->r_key, e->r_counts, e->r_cursor, e->r_off, e->r_perm, e->r_pos,
-                  e->r_fs, e->r_cnt, e->r_width, e->r_size, e->r_soff, e->r_col, e->r_blk,
+                  e->r_cpos, e->r_fs, e->r_cnt, e->r_width, e->r_size, e->r_soff, e->r_col, e->r_blk,
                   e->h_in, e->h_out, e->dP, e->dind, e->iperm, e->r_iperm, e->bigq,
                   e->rad, e->r_rad, e->selfv, e->tmp};
   for (void *p : ptrs)
@@ -369,6 +370,7 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
     E_TRY(hipMalloc(&e->r_perm, sizeof(int) * N));
     E_TRY(hipMalloc(&e->r_iperm, sizeof(int) * N));
     E_TRY(hipMalloc(&e->r_pos, sizeof(double) * 3 * N));
+    if (e->tsz == 4) E_TRY(hipMalloc(&e->r_cpos, sizeof(float4) * N));
     E_TRY(hipMalloc(&e->r_fs, e->tsz * (size_t)3 * e->max_rhs * N));
     e->nslices = (N + 63) / 64;
     E_TRY(hipMalloc(&e->r_cnt, sizeof(int) * N));
@@ -418,6 +420,7 @@ int build_P(_spme_t *e, hipStream_t st) {
       return fail(-2, "cell sort launch failed");
     launch_invert(e->r_perm, e->r_iperm, N, st);
     launch_cell_gather(e->r_perm, e->pos, e->r_pos, e->poly ? e->rad : nullptr, e->r_rad, N, st);
+    if (e->r_cpos) launch_cell_rel(e->r_pos, N, e->prm.box_size, e->nc, e->r_cpos, st);
     if (e->poly) launch_self_radii<T>(e->r_rad, N, e->prm.xi, (T *)e->selfv, st);
     const double L = e->prm.box_size, rc = e->prm.rc;
     const double cut = e->cut;
@@ -452,7 +455,7 @@ int build_P(_spme_t *e, hipStream_t st) {
     e->r_used = total;
     launch_pair_fill<T>(e->r_pos, e->r_off, e->nc, L, e->prm.xi, rc * rc, cut * cut, N,
                         e->r_width, e->r_soff, e->r_col, e->poly ? e->r_rad : nullptr,
-                        (T *)e->r_blk, e->r_E, e->err, st);
+                        (T *)e->r_blk, e->err, st);
   }
   HIP_TRY(hipGetLastError());
   return 0;
@@ -704,7 +707,7 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
   }
   if (parts & SPME_REAL) {
     if (!both) {
-      launch_permute_in<T>(e->r_iperm, (T *)e->r_fs, nullptr, nullptr, in, ldin, N, C, st);
+      launch_permute_in<T>(nullptr, nullptr, e->r_iperm, (T *)e->r_fs, in, ldin, N, C, st);
       mark(SPME_ST_PERMUTE_REAL);
     }
     const double xi = e->prm.xi;
@@ -712,10 +715,11 @@ int apply_t(_spme_t *e, int nrhs, double alpha, const T *in, long ldin, double b
                             ? 1.0 - 6.0 * xi / std::sqrt(M_PI) +
                                   40.0 * xi * xi * xi / (3.0 * std::sqrt(M_PI))
                             : 0.0;
-    RealArgs<T, TO> ra{e->r_perm, e->r_width, e->r_soff, e->r_col, (const T *)e->r_blk, e->r_E,
-                   e->r_pos, e->prm.box_size, self, e->poly ? (const T *)e->selfv : nullptr,
-                   (const T *)e->r_fs, N, C, alpha, beta,
-                   (parts & SPME_RECIP) ? 1 : 0, out, ldout};
+    RealArgs<T, TO> ra{e->r_perm, e->r_width, e->r_soff, e->r_col, (const T *)e->r_blk,
+                       e->r_pos, e->r_cpos, e->nc, float(e->prm.box_size / e->nc),
+                       e->prm.box_size, self, e->poly ? (const T *)e->selfv : nullptr,
+                       (const T *)e->r_fs, N, C, alpha, beta,
+                       (parts & SPME_RECIP) ? 1 : 0, out, ldout};
     if (launch_real(ra, st)) return fail(-1, "real space: unsupported batch width");
     mark(SPME_ST_REAL);
   }
@@ -1003,8 +1007,8 @@ int group_build_args(_spme_group_t *g, double alpha, const void *const *in, cons
                                                   40.0 * xi * xi * xi / (3.0 * std::sqrt(M_PI))
                                             : 0.0;
     ra[s] = RealArgs<T>{e->r_perm, e->r_width, e->r_soff, e->r_col, (const T *)e->r_blk,
-                        e->r_E, e->r_pos, e->prm.box_size, self, nullptr, (const T *)e->r_fs,
-                        e->N, C,
+                        e->r_pos, e->r_cpos, e->nc, float(e->prm.box_size / e->nc),
+                        e->prm.box_size, self, nullptr, (const T *)e->r_fs, e->N, C,
                         alpha, beta, rec ? 1 : 0, (T *)out[s], ldout[s]};
   }
   HIP_TRY(hipMemcpyAsync(g->d_args, g->h_args, bytes, hipMemcpyHostToDevice, st));

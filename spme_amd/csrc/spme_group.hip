@@ -47,7 +47,7 @@ __global__ __launch_bounds__(GROUP_ELEMS) void kg_permute(GroupStage<GPermArgs<T
     const long e = (long)lb * GROUP_ELEMS + threadIdx.x;
     if (e < (long)a.N * a.C) {
       if (a.perm) permute_elem<T>(a.perm, a.in, a.ldin, a.fs, a.C, e);
-      if (a.rperm) permute_elem<T>(a.rperm, a.in, a.ldin, a.rfs, a.C, e);
+      if (a.rperm) permute_elem_cell<T>(a.rperm, a.in, a.ldin, a.rfs, a.C, e);
     }
   });
 }
@@ -84,8 +84,9 @@ __global__ __launch_bounds__(GROUP_ELEMS) void kg_real(GroupStage<RealArgs<T>> t
     const long gi = (long)lb * GROUP_ELEMS + threadIdx.x;
     const int row = int(gi / LPR), l = int(gi % LPR);
     if (row < a.N)
-      real_lane<T, C>(a.perm, a.width, a.soff, a.col, a.blk, a.E, a.spos, a.L, T(a.self),
-                      a.selfv, a.fs, T(a.alpha), T(a.beta), a.accum, a.out, a.ldout, row, l);
+      real_lane<T, C>(a.perm, a.width, a.soff, a.col, a.blk, a.spos, a.cpos, a.nc, a.h, a.L,
+                      T(a.self), a.selfv, a.fs, T(a.alpha), T(a.beta), a.accum, a.out, a.ldout,
+                      row, l);
   });
 }
 

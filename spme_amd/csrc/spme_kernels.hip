@@ -169,8 +169,10 @@ __global__ __launch_bounds__(256) void k_pgather(const int *__restrict__ perm,
   pw[t] = T(w_tmp[(size_t)i * P3 + j]);
 }
 
-// Forces into sorted particle-major order, one or two orders in one pass:
-// fa[ia[i]*C + 3r + d] = in[r*ldin + 3i + d] (and fb through ib), C <= 48.
+// Forces into sorted order, one or two orders in one pass: for spreading,
+// particle-major fa[ia[i]*C + 3r + d] = in[r*ldin + 3i + d]; for the
+// real-space SpMV (cell order, the right-hand sides of one component
+// together) fb[ib[i]*C + d*(C/3) + r].  Either may be null; C <= 48.
 // With radii (rad != null, unequal-radius engines, C <= 24) fa takes six
 // components per right-hand side, f and a_i^2 f:
 // fa[ia[i]*2C + 6r + d] = f, fa[ia[i]*2C + 6r + 3 + d] = a_i^2 f.
@@ -200,19 +202,22 @@ __global__ __launch_bounds__(256) void k_permute_in(const int *__restrict__ ia,
     }
   }
   __syncthreads();
+  const int R = C / 3;
   for (int e = tid; e < n * C; e += 256) {
     const int i = e / C, c = e - i * C;
     const T v = s[i * CP + c];
-    if (rad) {
-      const int r = c / 3, d = c - 3 * r;
-      const double a = rad[i0 + i];
-      T *o = fa + (size_t)ia[i0 + i] * (2 * C) + 6 * r + d;
-      o[0] = v;
-      o[3] = T(a * a) * v;
-    } else {
-      fa[(size_t)ia[i0 + i] * C + c] = v;
+    const int r = c / 3, d = c - 3 * r;
+    if (fa) {
+      if (rad) {
+        const double a = rad[i0 + i];
+        T *o = fa + (size_t)ia[i0 + i] * (2 * C) + 6 * r + d;
+        o[0] = v;
+        o[3] = T(a * a) * v;
+      } else {
+        fa[(size_t)ia[i0 + i] * C + c] = v;
+      }
     }
-    if (fb) fb[(size_t)ib[i0 + i] * C + c] = v;
+    if (fb) fb[(size_t)ib[i0 + i] * C + d * R + r] = v;
   }
 }
 

@@ -125,6 +125,29 @@ __global__ __launch_bounds__(256) void k_cell_key(const double *__restrict__ pos
   atomicAdd(&counts[k], 1);
 }
 
+// FP32 engines' real-space positions (cell order): the offset of each
+// particle from the corner of its cell (the cell of cell_of, side h = L/nc),
+// formed in FP64 and rounded to FP32, and the cell's coordinates packed as
+// cx | cy << 10 | cz << 20 (nc <= 128).
+__global__ __launch_bounds__(256) void k_cell_rel(const double *__restrict__ spos, int N,
+                                                  double L, int nc, float4 *__restrict__ cpos) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= N) return;
+  const double h = L / nc;
+  float o[3];
+  int key = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double t = spos[3L * s + d] / L;
+    t -= floor(t);
+    int v = int(t * nc);
+    v = v >= nc ? nc - 1 : (v < 0 ? 0 : v);
+    o[d] = float(t * L - v * h);
+    key |= v << (10 * d);
+  }
+  cpos[s] = make_float4(o[0], o[1], o[2], __int_as_float(key));
+}
+
 // positions (and radii, when rad != null) into cell order
 __global__ __launch_bounds__(256) void k_cell_gather(const int *__restrict__ perm,
                                                      const double *__restrict__ pos,
@@ -253,7 +276,7 @@ __global__ __launch_bounds__(256) void k_pair_coeffs(const double *__restrict__ 
                                                      const int *__restrict__ soff,
                                                      const int *__restrict__ col,
                                                      const double *__restrict__ srad,
-                                                     T *__restrict__ blk, long E) {
+                                                     T *__restrict__ blk) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N) return;
   const int q = s >> 6, l = s & 63;
@@ -276,8 +299,8 @@ __global__ __launch_bounds__(256) void k_pair_coeffs(const double *__restrict__ 
       else
         pair_coeffs(r2, xi, rc2, F, G);
     }
-    blk[e] = T(F);
-    blk[E + e] = T(G / r2);
+    blk[2 * e] = T(F);
+    blk[2 * e + 1] = T(G / r2);
   }
 }
 
@@ -287,8 +310,10 @@ __global__ __launch_bounds__(256) void k_real_spmv(const int *__restrict__ perm,
                                                    const int *__restrict__ width,
                                                    const int *__restrict__ soff,
                                                    const int *__restrict__ col,
-                                                   const T *__restrict__ blk, long E,
-                                                   const double *__restrict__ spos, double L,
+                                                   const T *__restrict__ blk,
+                                                   const double *__restrict__ spos,
+                                                   const float4 *__restrict__ cpos, int nc,
+                                                   float h, double L,
                                                    T self, const T *__restrict__ selfv,
                                                    const T *__restrict__ fs, int N,
                                                    TO alpha, TO beta, int accum,
@@ -297,8 +322,8 @@ __global__ __launch_bounds__(256) void k_real_spmv(const int *__restrict__ perm,
   const long g = (long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int s = int(g / LPR), l = int(g % LPR);
   if (s < N)
-    real_lane<T, C, TO>(perm, width, soff, col, blk, E, spos, L, self, selfv, fs, alpha, beta,
-                        accum, out, ldout, s, l);
+    real_lane<T, C, TO>(perm, width, soff, col, blk, spos, cpos, nc, h, L, self, selfv, fs,
+                        alpha, beta, accum, out, ldout, s, l);
 }
 
 static int grid1(long n, int b) { return int((n + b - 1) / b); }
@@ -322,6 +347,10 @@ void launch_self_radii(const double *srad, int N, double xi, T *selfv, hipStream
 template void launch_self_radii<float>(const double *, int, double, float *, hipStream_t);
 template void launch_self_radii<double>(const double *, int, double, double *, hipStream_t);
 
+void launch_cell_rel(const double *spos, int N, double L, int nc, float4 *cpos, hipStream_t st) {
+  hipLaunchKernelGGL(k_cell_rel, dim3(grid1(N, 256)), dim3(256), 0, st, spos, N, L, nc, cpos);
+}
+
 void launch_pair_count(const double *spos, const int *coff, int nc, double L, double cut2,
                        int N, int *cnt, hipStream_t st) {
   hipLaunchKernelGGL(k_pair_count, dim3(grid1(N, 256)), dim3(256), 0, st, spos, coff, nc, L,
@@ -335,19 +364,19 @@ void launch_slice_width(const int *cnt, int N, int *width, int *size, hipStream_
 template <typename T>
 void launch_pair_fill(const double *spos, const int *coff, int nc, double L, double xi,
                       double rc2, double cut2, int N, const int *width, const int *soff,
-                      int *col, const double *srad, T *blk, long E, int *err, hipStream_t st) {
+                      int *col, const double *srad, T *blk, int *err, hipStream_t st) {
   hipLaunchKernelGGL(k_pair_cols, dim3(grid1(N, 256)), dim3(256), 0, st, spos, coff, nc, L,
                      cut2, N, width, soff, col, err);
   hipLaunchKernelGGL(k_pair_coeffs<T>, dim3(grid1(N, 256)), dim3(256), 0, st, spos, N, L, xi,
-                     rc2, width, soff, col, srad, blk, E);
+                     rc2, width, soff, col, srad, blk);
 }
 
 template <typename T, int C, typename TO>
 static void real_c(const RealArgs<T, TO> &a, hipStream_t st) {
   hipLaunchKernelGGL((k_real_spmv<T, C, TO>), dim3(grid1((long)a.N * RealShape<C>::LPR, 256)),
-                     dim3(256), 0, st, a.perm, a.width, a.soff, a.col, a.blk, a.E, a.spos, a.L,
-                     T(a.self), a.selfv, a.fs, a.N, TO(a.alpha), TO(a.beta), a.accum, a.out,
-                     a.ldout);
+                     dim3(256), 0, st, a.perm, a.width, a.soff, a.col, a.blk, a.spos, a.cpos,
+                     a.nc, a.h, a.L, T(a.self), a.selfv, a.fs, a.N, TO(a.alpha), TO(a.beta),
+                     a.accum, a.out, a.ldout);
 }
 
 template <typename T, typename TO>
@@ -364,10 +393,10 @@ int launch_real(const RealArgs<T, TO> &a, hipStream_t st) {
 
 template void launch_pair_fill<float>(const double *, const int *, int, double, double, double,
                                       double, int, const int *, const int *, int *,
-                                      const double *, float *, long, int *, hipStream_t);
+                                      const double *, float *, int *, hipStream_t);
 template void launch_pair_fill<double>(const double *, const int *, int, double, double, double,
                                        double, int, const int *, const int *, int *,
-                                       const double *, double *, long, int *, hipStream_t);
+                                       const double *, double *, int *, hipStream_t);
 template int launch_real<float, float>(const RealArgs<float, float> &, hipStream_t);
 template int launch_real<float, double>(const RealArgs<float, double> &, hipStream_t);
 template int launch_real<double, double>(const RealArgs<double, double> &, hipStream_t);
