@@ -1117,7 +1117,7 @@ __device__ __forceinline__ void real_lane(const int *__restrict__ perm,
   // U gathers are issued before any of them is used.  Entries past the row's
   // width repeat its last entry with zero coefficients (they add exact
   // zeros), so no load leaves the slice.
-  constexpr int U = 4;
+  constexpr int U = 6;
   int jn[U];
   Pair gn[U];
   auto fetch_entries = [&](int k0) {
