@@ -325,6 +325,12 @@ def main():
     t0 = time.perf_counter()
     eng.set_positions(pos_d)
     torch.cuda.synchronize()
+    t_pbuild_first = time.perf_counter() - t0
+    # the same build again: the first call of a process also loads the code
+    # objects of the build kernels
+    t0 = time.perf_counter()
+    eng.set_positions(pos_d)
+    torch.cuda.synchronize()
     t_pbuild = time.perf_counter() - t0
     tiles = eng.autotune(nrhs=R, reps=3)
     real_entries = eng.info().real_entries
@@ -532,6 +538,7 @@ def main():
                          "p_build_ms": round(pb_compact * 1e3, 2),
                          "speedup_of_batched_fp32": round(value / compact_value, 2)},
         "p_build_ms": round(t_pbuild * 1e3, 2),
+        "p_build_first_call_ms": round(t_pbuild_first * 1e3, 2),
         "rel_error_vs_fp64_direct_ewald": errors,
         "multi_system": multi,
         "brownian": brownian,

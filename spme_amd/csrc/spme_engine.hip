@@ -163,6 +163,7 @@ struct _spme_t {
   int *iperm, *r_iperm;  // inverses of perm and r_perm (original -> sorted)
   double *r_pos;
   float4 *r_cpos;  // FP32 engines: cell-relative positions for the SpMV
+  int *scan_sums;  // scratch of the device scans (1024 tile totals)
   void *r_fs;
   // real-space pair blocks (sliced ELL, 64 rows per slice)
   int nslices;
@@ -270,7 +271,7 @@ void free_all(_spme_t *e) {
                   e->r_key, e->r_counts, e->r_cursor, e->r_off, e->r_perm, e->r_pos,
                   e->r_cpos, e->r_fs, e->r_cnt, e->r_width, e->r_size, e->r_soff, e->r_col, e->r_blk,
                   e->h_in, e->h_out, e->dP, e->dind, e->iperm, e->r_iperm, e->bigq,
-                  e->rad, e->r_rad, e->selfv, e->tmp};
+                  e->rad, e->r_rad, e->selfv, e->tmp, e->scan_sums};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (auto &r : e->pending) {
@@ -331,6 +332,7 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
   E_TRY(hipMalloc(&e->cursor, sizeof(int) * e->nkeys));
   E_TRY(hipMalloc(&e->bigq, sizeof(int) * bucket_queue_ints(N)));
   E_TRY(hipMalloc(&e->boff, sizeof(int) * (e->nkeys + 1)));
+  E_TRY(hipMalloc(&e->scan_sums, sizeof(int) * 1024));
   E_TRY(hipMalloc(&e->perm, sizeof(int) * N));
   E_TRY(hipMalloc(&e->iperm, sizeof(int) * N));
   E_TRY(hipMalloc(&e->pbase, sizeof(int) * N));
@@ -402,7 +404,7 @@ int build_P(_spme_t *e, hipStream_t st) {
                        st))
       return fail(-1, "dense P: unsupported porder");
   } else {
-    launch_scan(e->counts, e->boff, e->nkeys, st);
+    launch_scan(e->counts, e->boff, e->nkeys, e->scan_sums, st);
     launch_pscatter(e->key_tmp, e->boff, e->cursor, e->perm, N, st);
     if (launch_bucket_sort(e->key_tmp, N, e->boff, e->perm, e->nkeys, e->bigq, e->iperm, st))
       return fail(-2, "bucket sort launch failed");
@@ -414,7 +416,7 @@ int build_P(_spme_t *e, hipStream_t st) {
     HIP_TRY(hipMemsetAsync(e->r_counts, 0, sizeof(int) * nc3, st));
     HIP_TRY(hipMemsetAsync(e->r_cursor, 0, sizeof(int) * nc3, st));
     launch_cell_key(e->pos, N, e->prm.box_size, e->nc, e->r_key, e->r_counts, st);
-    launch_scan(e->r_counts, e->r_off, nc3, st);
+    launch_scan(e->r_counts, e->r_off, nc3, e->scan_sums, st);
     launch_pscatter(e->r_key, e->r_off, e->r_cursor, e->r_perm, N, st);
     if (launch_bucket_sort(e->r_key, N, e->r_off, e->r_perm, nc3, e->bigq, e->r_iperm, st))
       return fail(-2, "cell sort launch failed");
@@ -439,7 +441,7 @@ int build_P(_spme_t *e, hipStream_t st) {
                   "real-space block list needs %ld entries (at most %d): the particles are "
                   "too clustered for rc = %g",
                   total, INT_MAX, rc);
-    launch_scan(e->r_size, e->r_soff, e->nslices, st);
+    launch_scan(e->r_size, e->r_soff, e->nslices, e->scan_sums, st);
     if (total > e->r_cap) {
       (void)hipFree(e->r_col);
       (void)hipFree(e->r_blk);

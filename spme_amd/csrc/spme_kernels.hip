@@ -59,29 +59,79 @@ __global__ __launch_bounds__(256) void k_pbuild(const double *__restrict__ pos, 
   atomicAdd(&counts[key], 1);
 }
 
-// Exclusive scan of n ints into out[0..n] (out[n] = total), one block.
-__global__ __launch_bounds__(1024) void k_scan(const int *__restrict__ in, int *__restrict__ out,
-                                               int n) {
-  __shared__ int part[1024];
-  const int t = threadIdx.x;
-  const int chunk = (n + 1023) / 1024;
-  const int a = min(n, t * chunk), e = min(n, a + chunk);
-  int s = 0;
-  for (int j = a; j < e; ++j) s += in[j];
-  part[t] = s;
+// Exclusive scan of n ints into out[0..n] (out[n] = total) in three passes
+// over tiles of SCAN_TILE values (1024 threads, 4 consecutive values each):
+// tile totals, a one-block scan of the totals, then each tile scanned again
+// from its offset.  n <= 1024 * SCAN_TILE.
+constexpr int SCAN_TILE = 4096;
+
+// Inclusive scan over the block of one value per thread (1024 threads);
+// returns the thread's inclusive prefix, and the block total in *tot.
+__device__ __forceinline__ int block_scan_1024(int v, int *s_w, int *tot) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    v += lane >= o ? u : 0;
+  }
+  if (lane == 63) s_w[w] = v;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = t >= off ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+  if (threadIdx.x < 16) {
+    int x = s_w[threadIdx.x];
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      const int u = __shfl_up(x, o, 16);
+      x += (threadIdx.x & 15) >= o ? u : 0;
+    }
+    s_w[16 + threadIdx.x] = x;
   }
-  int run = part[t] - s;
-  for (int j = a; j < e; ++j) {
-    out[j] = run;
-    run += in[j];
+  __syncthreads();
+  *tot = s_w[31];
+  return v + (w > 0 ? s_w[16 + w - 1] : 0);
+}
+
+__device__ __forceinline__ void scan_load4(const int *__restrict__ in, int n, long i0, int (&v)[4]) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) v[k] = i0 + k < n ? in[i0 + k] : 0;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(const int *__restrict__ in, int n,
+                                                     int *__restrict__ sums) {
+  __shared__ int s_w[32];
+  int v[4];
+  scan_load4(in, n, (long)blockIdx.x * SCAN_TILE + 4 * threadIdx.x, v);
+  int tot;
+  block_scan_1024(v[0] + v[1] + v[2] + v[3], s_w, &tot);
+  if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+// sums[0..nt) -> exclusive offsets in place (nt <= 1024); out[n] = total
+__global__ __launch_bounds__(1024) void k_scan_top(int *__restrict__ sums, int nt,
+                                                   int *__restrict__ out, int n) {
+  __shared__ int s_w[32];
+  const int t = threadIdx.x;
+  const int x = t < nt ? sums[t] : 0;
+  int tot;
+  const int inc = block_scan_1024(x, s_w, &tot);
+  if (t < nt) sums[t] = inc - x;
+  if (t == 0) out[n] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_out(const int *__restrict__ in, int n,
+                                                   const int *__restrict__ sums,
+                                                   int *__restrict__ out) {
+  __shared__ int s_w[32];
+  const long i0 = (long)blockIdx.x * SCAN_TILE + 4 * threadIdx.x;
+  int v[4];
+  scan_load4(in, n, i0, v);
+  const int own = v[0] + v[1] + v[2] + v[3];
+  int tot;
+  int run = block_scan_1024(own, s_w, &tot) - own + sums[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (i0 + k < n) out[i0 + k] = run;
+    run += v[k];
   }
-  if (t == 1023) out[n] = part[1023];
 }
 
 __global__ __launch_bounds__(256) void k_pscatter(const int *__restrict__ key,
@@ -179,9 +229,11 @@ __global__ __launch_bounds__(256) void k_pgather(const int *__restrict__ perm,
 // A block stages PB consecutive original particles in LDS (each input row
 // read once, coalesced), then writes each particle's C values as one
 // contiguous record per order.
+// 64 particles per block keep many blocks in flight (at N = 200k, 16 RHS,
+// 256 per block took 0.100 ms, 64 take 0.042 ms).
 template <typename T> struct PermuteShape {
-  static constexpr int PB = sizeof(T) == 8 ? 128 : 256;  // particles per block
-  static constexpr int CP = 48 + 1;                       // LDS pitch per particle
+  static constexpr int PB = 64;      // particles per block
+  static constexpr int CP = 48 + 1;  // LDS pitch per particle
 };
 
 template <typename T>
@@ -768,8 +820,16 @@ void launch_pbuild(const double *pos, int N, double L, int K, int P, const WTabl
                      TY, base_tmp, w_tmp, key_tmp, counts, err);
 }
 
-void launch_scan(const int *in, int *out, int n, hipStream_t st) {
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, in, out, n);
+int scan_tiles(int n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+// sums: scan_tiles(n) ints of scratch
+void launch_scan(const int *in, int *out, int n, int *sums, hipStream_t st) {
+  const int nt = scan_tiles(n);
+  if (nt > 0)
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(1024), 0, st, in, n, sums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, sums, nt, out, n);
+  if (nt > 0)
+    hipLaunchKernelGGL(k_scan_out, dim3(nt), dim3(1024), 0, st, in, n, sums, out);
 }
 
 void launch_pscatter(const int *key, const int *boff, int *cursor, int *perm, int N,
