@@ -13,8 +13,9 @@ pick (xi, K, p, rc):
 Each is run as the full M.f in FP64 with one right-hand side, the form the
 training labels were measured in: the relative error against the direct rows
 and the apply time.  The test checks that the learned selector meets the
-tolerance at least as often as the plain analytic one, that at equal
-reliability (against analytic/4) it is faster overall, and that at
+tolerance at least as often as the plain analytic one and as the analytic
+one with its safety factor (analytic/4), that it is faster overall than
+analytic/4, and that at
 N = 500,000 it acts (no fallback) and meets both tolerances, also in that
 case's evaluation form: 8 right-hand sides batched, FP64 (and FP32 at 1e-4).
 """
@@ -106,5 +107,5 @@ def test_learned_selector_gain():
         print(f"{k:11s} meets tolerance {ok}/{total}, total time {ms:.2f} ms", flush=True)
     assert len(large) == 2 * len(TOLS) + 1 and all(h == "learned" and ok for h, ok in large), large
     assert summ["learned"][0] >= summ["analytic"][0]
-    assert summ["learned"][0] >= summ["analytic/4"][0] - 1
+    assert summ["learned"][0] >= summ["analytic/4"][0]
     assert summ["learned"][1] < summ["analytic/4"][1]
