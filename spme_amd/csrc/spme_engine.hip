@@ -321,7 +321,10 @@ int engine_create(const spme_params_t *prm, _spme_t **out) {
   // the plain path transforms one right-hand side (3 components) at a time
   E_TRY(hipMalloc(&e->grid, (size_t)(e->dense ? 3 : C) * e->comp_stride * e->tsz));
   const bool fused = e->tsz == 4 ? fft_fused<float>(K) : fft_fused<double>(K);
-  if (!e->dense && fused) E_TRY(hipMalloc(&e->nyq, (size_t)C * K * K * 2 * e->tsz));
+  // a separate Nyquist column array for the large fused meshes only (the
+  // grouped kernels, which serve K <= 96, keep it in the spectrum rows)
+  if (!e->dense && fused && K >= 128)
+    E_TRY(hipMalloc(&e->nyq, (size_t)C * K * K * 2 * e->tsz));
   int rc = e->tsz == 4 ? upload_tables<float>(e) : upload_tables<double>(e);
   if (rc) return bail(rc);
   e->nkeys = (K / 8) * (K / 8) * K;

@@ -44,10 +44,19 @@ inline int group_perm_blocks(int N, int C) {
 inline int group_real_blocks(int N, int C) {
   return int(((long)N * real_lpr(C) + GROUP_ELEMS - 1) / GROUP_ELEMS);
 }
+// Meshes whose grouped x passes are the fused x + y plane passes of the
+// single-system path (256 threads per block, as their y passes).
+template <typename T> constexpr bool group_xy(int K) { return K <= 96 && xy_fused<T>(K); }
+
 // Logical blocks of FFT pass i (0 x fwd, 1 y fwd, 2 z + influence,
-// 3 y inv, 4 x inv) of a K^3 mesh with ncomp components.
+// 3 y inv, 4 x inv) of a K^3 mesh with ncomp components; with fused x + y
+// planes, passes 0 and 4 take one block per plane and passes 1 and 3 none.
 template <typename T> inline int fft_pass_blocks(int K, int ncomp, int pass) {
   using S = FftShape<T>;
+  if (group_xy<T>(K)) {
+    if (pass == 0 || pass == 4) return ncomp * K;
+    if (pass == 1 || pass == 3) return 0;
+  }
   if (pass == 0 || pass == 4) {
     const long npairs = (long)ncomp * K * K / 2;
     return int((npairs + S::NLX - 1) / S::NLX);

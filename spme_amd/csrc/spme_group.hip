@@ -97,10 +97,21 @@ __global__ __launch_bounds__(GROUP_ELEMS) void kg_real(GroupStage<RealArgs<T>> t
 // doubles their register use).
 #define SPME_FOR_EACH_K(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
 
+// Pass 0 (x forward) and pass 4 (x inverse).  Meshes with K <= 96 run the
+// fused x + y plane bodies of the single-system path (one (component, z)
+// plane per logical block; their y passes have no blocks), larger ones the
+// separate x pass.  LDS holds the larger of the two buffers.
+template <typename T, int KMAX>
+constexpr int group_x_buf() {
+  constexpr int NL = FftShape<T>::NLX;
+  constexpr int KS = KMAX < 96 ? KMAX : 96;
+  return NL * (KMAX + 1) > XYShape<T, KS>::NCPX ? NL * (KMAX + 1) : XYShape<T, KS>::NCPX;
+}
+
 template <typename T, int KMAX, int NT>
 __global__ __launch_bounds__(NT) void kg_fft_x_r2c(GroupStage<FftArgs<T>> t) {
   constexpr int NL = FftShape<T>::NLX;
-  __shared__ cpx<T> buf[NL * (KMAX + 1)];
+  __shared__ cpx<T> buf[group_x_buf<T, KMAX>()];
   __shared__ cpx<T> s_tw[KMAX];
   {
     const int g = blockIdx.x;
@@ -110,10 +121,15 @@ __global__ __launch_bounds__(NT) void kg_fft_x_r2c(GroupStage<FftArgs<T>> t) {
     const FftArgs<T> &a = t.args[s];
     const long npairs = (long)a.ncomp * a.K * a.K / 2;
     switch (a.K) {
-#define X(KK)                                                                        \
-  case KK:                                                                           \
-    if constexpr (KK <= KMAX)                                                        \
-      fft_x_r2c_block<T, KK, NL, NT>(a.grid, a.RS, npairs, a.tw, lb, buf, s_tw);     \
+#define X(KK)                                                                          \
+  case KK:                                                                             \
+    if constexpr (KK <= KMAX) {                                                        \
+      if constexpr (group_xy<T>(KK))                                                   \
+        xy_fwd_block<T, KK, NT, false>(a.grid, a.RS, a.comp_stride, a.ncomp * KK, a.tw, \
+                                       nullptr, lb, a.ncomp * KK, buf, s_tw);          \
+      else                                                                             \
+        fft_x_r2c_block<T, KK, NL, NT>(a.grid, a.RS, npairs, a.tw, lb, buf, s_tw);     \
+    }                                                                                  \
     break;
       SPME_FOR_EACH_K(X)
 #undef X
@@ -124,7 +140,7 @@ __global__ __launch_bounds__(NT) void kg_fft_x_r2c(GroupStage<FftArgs<T>> t) {
 template <typename T, int KMAX, int NT>
 __global__ __launch_bounds__(NT) void kg_fft_x_c2r(GroupStage<FftArgs<T>> t) {
   constexpr int NL = FftShape<T>::NLX;
-  __shared__ cpx<T> buf[NL * (KMAX + 1)];
+  __shared__ cpx<T> buf[group_x_buf<T, KMAX>()];
   __shared__ cpx<T> s_tw[KMAX];
   {
     const int g = blockIdx.x;
@@ -134,10 +150,15 @@ __global__ __launch_bounds__(NT) void kg_fft_x_c2r(GroupStage<FftArgs<T>> t) {
     const FftArgs<T> &a = t.args[s];
     const long npairs = (long)a.ncomp * a.K * a.K / 2;
     switch (a.K) {
-#define X(KK)                                                                        \
-  case KK:                                                                           \
-    if constexpr (KK <= KMAX)                                                        \
-      fft_x_c2r_block<T, KK, NL, NT>(a.grid, a.RS, npairs, a.tw, lb, buf, s_tw);     \
+#define X(KK)                                                                          \
+  case KK:                                                                             \
+    if constexpr (KK <= KMAX) {                                                        \
+      if constexpr (group_xy<T>(KK))                                                   \
+        xy_inv_block<T, KK, NT, false>(a.grid, a.RS, a.comp_stride, a.ncomp * KK, a.tw, \
+                                       nullptr, lb, a.ncomp * KK, buf, s_tw);          \
+      else                                                                             \
+        fft_x_c2r_block<T, KK, NL, NT>(a.grid, a.RS, npairs, a.tw, lb, buf, s_tw);     \
+    }                                                                                  \
     break;
       SPME_FOR_EACH_K(X)
 #undef X
@@ -240,6 +261,7 @@ template <typename T, int KMAX>
 static int group_fft_k(const GroupStage<FftArgs<T>> &t, int pass, hipStream_t st) {
   constexpr int NT = 256;
   const int grid = t.total;
+  if (grid == 0) return 0;  // the y passes of groups whose meshes all run fused
   switch (pass) {
     case 0: hipLaunchKernelGGL((kg_fft_x_r2c<T, KMAX, NT>), dim3(grid), dim3(NT), 0, st, t); break;
     case 1: hipLaunchKernelGGL((kg_fft_y<T, KMAX, -1, NT>), dim3(grid), dim3(NT), 0, st, t); break;

@@ -355,378 +355,26 @@ __global__ __launch_bounds__(NT) void k_fft_z_infl(T *__restrict__ grid, long co
 }
 
 // ---------------------------------------------------------------------------
-// Fused x + y transforms: one (component, z) plane at a time in LDS, so the
-// real mesh is read once and the half spectrum written once per direction
-// (the separate x and y passes move the spectrum through HBM twice more).
-// LDS holds K/2 line pairs of pitch LP = 2*Kh complex: during the x
-// transform line l is the complex row (row 2l) + i (row 2l+1); after the
-// split, row y of the half spectrum sits at y*Kh (2l*Kh = l*LP and
-// (2l+1)*Kh = l*LP + Kh), i.e. the plane is [y][x] with pitch Kh for the y
-// transform.  Blocks loop over planes and load the next plane into
-// registers while the second half of the current one is transformed and
-// stored.
-template <typename T, int K> struct XYShape {
-  static constexpr int Kh = K / 2 + 1;
-  static constexpr int LP = 2 * Kh;
-  static constexpr int NCPX = (K / 2) * LP;
-  static constexpr size_t LDS = (NCPX + K) * sizeof(cpx<T>);
-};
-
-// Split of packed lines into two half spectra, in place: for line l and
-// k <= K/2, A_k = (Z_k + conj Z_{K-k})/2 (row 2l), B_k = (Z_k - conj
-// Z_{K-k})/(2i) (row 2l+1).  Writing B_k (position Kh + k) clobbers Z at
-// K - (K/2 - 1 - k), so k and K/2 - 1 - k go to one thread.
-template <typename T, int K>
-__device__ __forceinline__ void xy_split(cpx<T> *line, int k) {
-  constexpr int Kh = K / 2 + 1;
-  const cpx<T> Z = line[k], Zm = cconj(line[k == 0 ? 0 : K - k]);
-  const cpx<T> A = {T(0.5) * (Z.x + Zm.x), T(0.5) * (Z.y + Zm.y)};
-  const cpx<T> D = csub(Z, Zm);
-  const cpx<T> B = {T(0.5) * D.y, T(-0.5) * D.x};
-  line[k] = A;
-  line[Kh + k] = B;
-}
-
-// Inverse of the split (packing two half spectra into one line of K),
-// dropping the imaginary parts at k = 0 and K/2 (real part of the IDFT).
-template <typename T, int K>
-__device__ __forceinline__ void xy_pack(cpx<T> *line, int k) {
-  constexpr int Kh = K / 2 + 1;
-  cpx<T> A = line[k], B = line[Kh + k];
-  if (k == 0 || k == K / 2) {
-    A.y = T(0);
-    B.y = T(0);
-  }
-  line[k] = {A.x - B.y, A.y + B.x};
-  if (k > 0 && k < K / 2) line[K - k] = {A.x + B.y, B.x - A.y};
-}
-
-// NYQ: the Nyquist column x = K/2 goes to its own array (nyq), which the
-// launcher passes exactly when NYQ is set.
+// Fused x + y transforms (bodies in spme_bodies.h): single-system kernels,
+// each block striding over the (component, z) planes.
 template <typename T, int K, int NT, bool NYQ>
 __global__ __launch_bounds__(NT) void k_fft_xy_fwd(T *__restrict__ grid, int RS, long comp_stride,
                                                    int nplanes, const cpx<T> *__restrict__ tw,
                                                    cpx<T> *__restrict__ nyq) {
-  using S = XYShape<T, K>;
-  constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
-  constexpr int Kh = S::Kh, LP = S::LP, L2 = K / 2;
-  constexpr int G = 16 / sizeof(T);  // reals per 16-byte load
-  constexpr int UPL = K / G, NUNIT = L2 * UPL, NU = (NUNIT + NT - 1) / NT;
-  __shared__ cpx<T> buf[S::NCPX];
+  __shared__ cpx<T> buf[XYShape<T, K>::NCPX];
   __shared__ cpx<T> s_tw[K];
-  using V = Vec16<T>;
-  load_tw<T, K>(s_tw, tw);
-  int tid = threadIdx.x;
-  const int Khp = RS / 2;
-  V ra[NU], rb[NU];
-  auto fetch = [&](int q) {
-    const int comp = q / K, z = q - comp * K;
-    const T *base = grid + comp * comp_stride + (size_t)z * K * RS;
-#pragma unroll
-    for (int r = 0; r < NU; ++r) {
-      const bool full = NT * (r + 1) <= NUNIT;  // a constant once unrolled
-      const int u = tid + NT * r;
-      if (full || u < NUNIT) {
-        const int l = u / UPL, g = u - l * UPL;
-        ra[r] = *reinterpret_cast<const V *>(base + (size_t)(2 * l) * RS + G * g);
-        rb[r] = *reinterpret_cast<const V *>(base + (size_t)(2 * l + 1) * RS + G * g);
-      }
-    }
-  };
-  auto commit = [&]() {
-#pragma unroll
-    for (int r = 0; r < NU; ++r) {
-      const bool full = NT * (r + 1) <= NUNIT;
-      const int u = tid + NT * r;
-      if (full || u < NUNIT) {
-        const int l = u / UPL, g = u - l * UPL;
-#pragma unroll
-        for (int i = 0; i < G; ++i) buf[l * LP + G * g + i] = {ra[r].v[i], rb[r].v[i]};
-      }
-    }
-  };
-  int q = blockIdx.x;
-  if (q < nplanes) fetch(q);
-  for (; q < nplanes; q += gridDim.x) {
-    // keep per-thread index arithmetic inside the loop: hoisted, it would
-    // stay live across the transform and spill
-    asm volatile("" : "+v"(tid));
-    commit();
-    __syncthreads();
-    // the next plane's loads, in flight through this one
-    if (q + (int)gridDim.x < nplanes) fetch(q + gridDim.x);
-    // x, step A: per line and n2, an N1-point FFT, twiddled, in place
-    for (int t = tid; t < L2 * N2; t += NT) {
-      const int l = t / N2, n2 = t - l * N2;
-      cpx<T> *ln = buf + l * LP;
-      cpx<T> a[N1];
-#pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) a[n1] = ln[n1 * N2 + n2];
-      reg_fft<N1, -1>(a);
-#pragma unroll
-      for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<-1>(s_tw, n2 * k1));
-#pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) ln[k1 * N2 + n2] = a[k1];
-    }
-    __syncthreads();
-    // x, step B, natural order: in rounds of whole lines (a line's tasks
-    // read and write the same positions), one register tile per thread
-    constexpr int LPR = NT / N1;  // lines per round
-    for (int l0 = 0; l0 < L2; l0 += LPR) {
-      const int l = l0 + tid / N1, k1 = tid % N1;
-      const bool on = tid < LPR * N1 && l < L2;
-      cpx<T> b[N2];
-      if (on) {
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) b[n2] = buf[l * LP + k1 * N2 + n2];
-        reg_fft<N2, -1>(b);
-      }
-      __syncthreads();
-      if (on) {
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) buf[l * LP + k1 + N1 * k2] = b[k2];
-      }
-      __syncthreads();
-    }
-    // split into half spectra: k and K/2-1-k per thread, K/2 alone
-    for (int t = tid; t < L2 * (K / 4 + 1); t += NT) {
-      const int l = t / (K / 4 + 1), m = t - l * (K / 4 + 1);
-      cpx<T> *ln = buf + l * LP;
-      if (m < K / 4) {
-        // both reads before either write
-        const int k0 = m, k1 = K / 2 - 1 - m;
-        const cpx<T> Z0 = ln[k0], Z0m = cconj(ln[k0 == 0 ? 0 : K - k0]);
-        const cpx<T> Z1 = ln[k1], Z1m = cconj(ln[K - k1]);
-        const cpx<T> D0 = csub(Z0, Z0m), D1 = csub(Z1, Z1m);
-        ln[k0] = {T(0.5) * (Z0.x + Z0m.x), T(0.5) * (Z0.y + Z0m.y)};
-        ln[Kh + k0] = {T(0.5) * D0.y, T(-0.5) * D0.x};
-        ln[k1] = {T(0.5) * (Z1.x + Z1m.x), T(0.5) * (Z1.y + Z1m.y)};
-        ln[Kh + k1] = {T(0.5) * D1.y, T(-0.5) * D1.x};
-      } else {
-        xy_split<T, K>(ln, K / 2);
-      }
-    }
-    __syncthreads();
-    // y, step A over the Kh columns of the [y][x] plane (pitch Kh)
-    for (int t = tid; t < Kh * N2; t += NT) {
-      const int x = t % Kh, n2 = t / Kh;
-      cpx<T> a[N1];
-#pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) a[n1] = buf[(n1 * N2 + n2) * Kh + x];
-      reg_fft<N1, -1>(a);
-#pragma unroll
-      for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<-1>(s_tw, n2 * k1));
-#pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) buf[(k1 * N2 + n2) * Kh + x] = a[k1];
-    }
-    __syncthreads();
-    // y, step B straight to the spectrum rows: the columns that stay in
-    // the rows (all Kh, or the K/2 below the Nyquist column), then, with a
-    // separate Nyquist array, that column's N1 tasks in the first lanes
-    {
-      const int comp = q / K, z = q - comp * K;
-      cpx<T> *spec = reinterpret_cast<cpx<T> *>(grid + comp * comp_stride) + (size_t)z * K * Khp;
-      constexpr int XR = NYQ ? K / 2 : Kh;
-      for (int t = tid; t < XR * N1; t += NT) {
-        const int x = t % XR, k1 = t / XR;
-        cpx<T> bb[N2];
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) bb[n2] = buf[(k1 * N2 + n2) * Kh + x];
-        reg_fft<N2, -1>(bb);
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) spec[(size_t)(k1 + N1 * k2) * Khp + x] = bb[k2];
-      }
-      if (NYQ && tid < N1) {
-        cpx<T> *nq = nyq + ((size_t)comp * K + z) * K;
-        const int k1 = tid;
-        cpx<T> bb[N2];
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) bb[n2] = buf[(k1 * N2 + n2) * Kh + K / 2];
-        reg_fft<N2, -1>(bb);
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) nq[k1 + N1 * k2] = bb[k2];
-      }
-    }
-    __syncthreads();
-  }
+  xy_fwd_block<T, K, NT, NYQ>(grid, RS, comp_stride, nplanes, tw, nyq, blockIdx.x, gridDim.x,
+                              buf, s_tw);
 }
 
-// NYQ: the Nyquist column x = K/2 comes from its own array (nyq), which
-// the launcher passes exactly when NYQ is set.
 template <typename T, int K, int NT, bool NYQ>
 __global__ __launch_bounds__(NT) void k_fft_xy_inv(T *__restrict__ grid, int RS, long comp_stride,
                                                    int nplanes, const cpx<T> *__restrict__ tw,
                                                    const cpx<T> *__restrict__ nyq) {
-  using S = XYShape<T, K>;
-  constexpr int N1 = Split<K>::N1, N2 = Split<K>::N2;
-  constexpr int Kh = S::Kh, LP = S::LP, L2 = K / 2;
-  constexpr int G = 16 / sizeof(cpx<T>);  // complex per 16-byte load
-  // units of G complex per plane: whole rows x < Kh (UPR per row), or, with
-  // a separate Nyquist column, rows x < K/2 (UPQ per row) and the column
-  // (K/G units)
-  constexpr int UPR = (Kh + G - 1) / G, UPQ = (K / 2) / G;
-  static_assert((K / 2) % G == 0 && K % G == 0, "whole units");
-  constexpr int NUNIT = NYQ ? K * UPQ + K / G : K * UPR;
-  constexpr int NU = (NUNIT + NT - 1) / NT;
-  __shared__ cpx<T> buf[S::NCPX];
+  __shared__ cpx<T> buf[XYShape<T, K>::NCPX];
   __shared__ cpx<T> s_tw[K];
-  struct alignas(16) C16 { cpx<T> v[G]; };
-  load_tw<T, K>(s_tw, tw);
-  int tid = threadIdx.x;
-  const int Khp = RS / 2;
-  C16 rv[NU];
-  constexpr int upr = NYQ ? UPQ : UPR;  // row units (a constant: divisions by it are cheap)
-  auto fetch = [&](int q) {
-    const int comp = q / K, z = q - comp * K;
-    const cpx<T> *spec =
-        reinterpret_cast<const cpx<T> *>(grid + comp * comp_stride) + (size_t)z * K * Khp;
-    const cpx<T> *nq = NYQ ? nyq + ((size_t)comp * K + z) * K : nullptr;
-#pragma unroll
-    for (int r = 0; r < NU; ++r) {
-      // full(r): every thread's unit r exists; rows(r): all of them are row
-      // units (constants once the loop is unrolled: no branch where true)
-      const bool full = NT * (r + 1) <= NUNIT, rows = NT * (r + 1) <= K * upr;
-      const int u = tid + NT * r;
-      if (full || u < NUNIT) {
-        const int y = u / upr, g = u - y * upr;
-        if (!NYQ || rows || y < K)
-          rv[r] = *reinterpret_cast<const C16 *>(spec + (size_t)y * Khp + G * g);
-        else
-          rv[r] = *reinterpret_cast<const C16 *>(nq + G * (u - K * upr));
-      }
-    }
-  };
-  auto commit = [&]() {
-#pragma unroll
-    for (int r = 0; r < NU; ++r) {
-      const bool full = NT * (r + 1) <= NUNIT, rows = NT * (r + 1) <= K * upr;
-      const int u = tid + NT * r;
-      if (full || u < NUNIT) {
-        const int y = u / upr, g = u - y * upr;
-        if (!NYQ || rows || y < K) {
-          // with the Nyquist column apart, row units cover x < K/2 < Kh
-#pragma unroll
-          for (int i = 0; i < G; ++i)
-            if (NYQ || G * g + i < Kh) buf[y * Kh + G * g + i] = rv[r].v[i];
-        } else {  // Nyquist units: rows G w .. G w + G - 1 of column K/2
-          const int w = u - K * upr;
-#pragma unroll
-          for (int i = 0; i < G; ++i) buf[(G * w + i) * Kh + K / 2] = rv[r].v[i];
-        }
-      }
-    }
-  };
-  int q = blockIdx.x;
-  if (q < nplanes) fetch(q);
-  for (; q < nplanes; q += gridDim.x) {
-    // keep per-thread index arithmetic inside the loop: hoisted, it would
-    // stay live across the transform and spill
-    asm volatile("" : "+v"(tid));
-    commit();
-    __syncthreads();
-    // the next plane's loads, in flight through this one
-    if (q + (int)gridDim.x < nplanes) fetch(q + gridDim.x);
-    // y inverse, step A in place
-    for (int t = tid; t < Kh * N2; t += NT) {
-      const int x = t % Kh, n2 = t / Kh;
-      cpx<T> a[N1];
-#pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) a[n1] = buf[(n1 * N2 + n2) * Kh + x];
-      reg_fft<N1, +1>(a);
-#pragma unroll
-      for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<+1>(s_tw, n2 * k1));
-#pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) buf[(k1 * N2 + n2) * Kh + x] = a[k1];
-    }
-    __syncthreads();
-    // y inverse, step B, natural order: in rounds of whole columns (a
-    // column's tasks read and write the same positions), one register tile
-    // per thread.  When the columns before the Nyquist column fill whole
-    // rounds, the Nyquist column x = K/2 is done in the last round by the
-    // first N1 lanes of wave 0 after their own column: its 16 tasks lie in
-    // one wave, and each task's writes wait on its reads through the FFT,
-    // so it needs no barrier of its own (a round of its own would cost two).
-    constexpr int CPR = NT / N1;  // columns per round
-    constexpr bool NYQ_IN_ROUND = (Kh - 1) % CPR == 0 && N1 <= 64;
-    constexpr int KX = NYQ_IN_ROUND ? Kh - 1 : Kh;  // columns done in rounds
-    for (int x0 = 0; x0 < KX; x0 += CPR) {
-      const int x = x0 + tid % CPR, k1 = tid / CPR;  // consecutive lanes: consecutive x
-      const bool on = tid < CPR * N1 && x < KX;
-      cpx<T> c[N2];
-      if (on) {
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) c[n2] = buf[(k1 * N2 + n2) * Kh + x];
-        reg_fft<N2, +1>(c);
-      }
-      __syncthreads();
-      if (on) {
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) buf[(k1 + N1 * k2) * Kh + x] = c[k2];
-      }
-      if (NYQ_IN_ROUND && x0 + CPR >= KX && tid < N1) {
-        const int kn = tid;  // task k1 of column K/2
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) c[n2] = buf[(kn * N2 + n2) * Kh + K / 2];
-        reg_fft<N2, +1>(c);
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) buf[(kn + N1 * k2) * Kh + K / 2] = c[k2];
-      }
-      __syncthreads();
-    }
-    // pack row pairs into lines: k and K/2-1-k per thread, K/2 alone
-    for (int t = tid; t < L2 * (K / 4 + 1); t += NT) {
-      const int l = t / (K / 4 + 1), m = t - l * (K / 4 + 1);
-      cpx<T> *ln = buf + l * LP;
-      if (m < K / 4) {
-        const int k0 = m, k1 = K / 2 - 1 - m;
-        cpx<T> A0 = ln[k0], B0 = ln[Kh + k0], A1 = ln[k1], B1 = ln[Kh + k1];
-        if (k0 == 0) {
-          A0.y = T(0);
-          B0.y = T(0);
-        }
-        ln[k0] = {A0.x - B0.y, A0.y + B0.x};
-        if (k0 > 0) ln[K - k0] = {A0.x + B0.y, B0.x - A0.y};
-        ln[k1] = {A1.x - B1.y, A1.y + B1.x};
-        ln[K - k1] = {A1.x + B1.y, B1.x - A1.y};
-      } else {
-        xy_pack<T, K>(ln, K / 2);
-      }
-    }
-    __syncthreads();
-    // x inverse, step A in place
-    for (int t = tid; t < L2 * N2; t += NT) {
-      const int l = t / N2, n2 = t - l * N2;
-      cpx<T> *ln = buf + l * LP;
-      cpx<T> a[N1];
-#pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) a[n1] = ln[n1 * N2 + n2];
-      reg_fft<N1, +1>(a);
-#pragma unroll
-      for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<+1>(s_tw, n2 * k1));
-#pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) ln[k1 * N2 + n2] = a[k1];
-    }
-    __syncthreads();
-    // x inverse, step B straight to the real rows 2l (real part), 2l+1 (imag)
-    {
-      const int comp = q / K, z = q - comp * K;
-      T *base = grid + comp * comp_stride + (size_t)z * K * RS;
-      for (int t = tid; t < L2 * N1; t += NT) {
-        const int l = t / N1, k1 = t - l * N1;
-        cpx<T> bb[N2];
-#pragma unroll
-        for (int n2 = 0; n2 < N2; ++n2) bb[n2] = buf[l * LP + k1 * N2 + n2];
-        reg_fft<N2, +1>(bb);
-        T *ra = base + (size_t)(2 * l) * RS;
-#pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) {
-          ra[k1 + N1 * k2] = bb[k2].x;
-          ra[RS + k1 + N1 * k2] = bb[k2].y;
-        }
-      }
-    }
-    __syncthreads();
-  }
+  xy_inv_block<T, K, NT, NYQ>(grid, RS, comp_stride, nplanes, tw, nyq, blockIdx.x, gridDim.x,
+                              buf, s_tw);
 }
 
 // ---------------------------------------------------------------------------
@@ -963,28 +611,18 @@ int launch_interp(const InterpArgs<T, TO> &a, hipStream_t st) {
   return -1;
 }
 
-// Fused x + y passes for meshes of K >= 128 whose planes fit in LDS (K 128
-// and 192 in FP32, 128 in FP64); otherwise separate x and y passes.  Smaller
-// meshes stay in the separate passes: their spectrum is cache-resident, and
-// the grouped kernels (which use the separate passes) then give the same
-// bits as single-system applies.
-template <typename T, int K> constexpr bool xy_fits() {
-  return K >= 128 && XYShape<T, K>::LDS <= 160 * 1024;
-}
+// Fused x + y passes wherever a plane fits in LDS (every K up to 192 in
+// FP32, up to 128 in FP64); otherwise separate x and y passes.  The grouped
+// kernels run the same plane bodies for K <= 96, so grouped and
+// single-system applies of those meshes give the same bits.
+template <typename T, int K> constexpr bool xy_fits() { return xy_fused<T>(K); }
 // Threads per plane block: enough registers per lane (168 at 768 threads)
 // for the next plane's prefetch beside the transform's register tiles.
 template <typename T, int K> constexpr int xy_threads() {
   return K >= 192 ? 768 : (K >= 128 ? (sizeof(T) == 8 ? 768 : 512) : 256);
 }
 
-template <typename T> bool fft_fused(int K) {
-  switch (K) {
-    case 128: return xy_fits<T, 128>();
-    case 192: return xy_fits<T, 192>();
-    case 256: return xy_fits<T, 256>();
-    default: return false;  // xy_fits needs K >= 128
-  }
-}
+template <typename T> bool fft_fused(int K) { return fft_size_supported(K) && xy_fused<T>(K); }
 template bool fft_fused<float>(int);
 template bool fft_fused<double>(int);
 
@@ -1019,7 +657,8 @@ static void fft3d_k(const FftArgs<T> &a, hipStream_t st, FftMarks *mk) {
   if constexpr (xy_fits<T, K>()) {
     constexpr int NTXY = xy_threads<T, K>();
     const int nplanes = a.ncomp * K;
-    const int per_cu = std::max(1, std::min(2, int((160 * 1024) / XYShape<T, K>::LDS)));
+    const int per_cu =
+        std::max(1, std::min(K >= 128 ? 2 : 4, int((160 * 1024) / XYShape<T, K>::LDS)));
     const int grid = std::min(nplanes, std::max(1, a.ncu) * per_cu);
     if (a.nyq)
       hipLaunchKernelGGL((k_fft_xy_fwd<T, K, NTXY, true>), dim3(grid), dim3(NTXY), 0, st, a.grid,

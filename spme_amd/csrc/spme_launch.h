@@ -113,6 +113,12 @@ template <typename T> struct FftArgs {
 };
 // True when the single-system apply of this mesh uses the fused x + y plane
 // passes (and then a separate Nyquist column).
+// LDS bytes of one fused x + y plane: K/2 line pairs of 2(K/2 + 1) complex
+// values, and the twiddles.
+template <typename T> constexpr size_t xy_lds(int K) {
+  return size_t((K / 2) * 2 * (K / 2 + 1) + K) * 2 * sizeof(T);
+}
+template <typename T> constexpr bool xy_fused(int K) { return xy_lds<T>(K) <= 160 * 1024; }
 template <typename T> bool fft_fused(int K);
 
 // Events recorded after each FFT launch and the pass each one ends
