@@ -407,7 +407,8 @@ int build_P(_spme_t *e, hipStream_t st) {
                        st))
       return fail(-1, "dense P: unsupported porder");
   } else {
-    launch_scan(e->counts, e->boff, e->nkeys, e->scan_sums, st);
+    if (launch_scan(e->counts, e->boff, e->nkeys, e->scan_sums, st))
+      return fail(-1, "scan of more than 2^22 values");
     launch_pscatter(e->key_tmp, e->boff, e->cursor, e->perm, N, st);
     if (launch_bucket_sort(e->key_tmp, N, e->boff, e->perm, e->nkeys, e->bigq, e->iperm, st))
       return fail(-2, "bucket sort launch failed");
@@ -419,7 +420,8 @@ int build_P(_spme_t *e, hipStream_t st) {
     HIP_TRY(hipMemsetAsync(e->r_counts, 0, sizeof(int) * nc3, st));
     HIP_TRY(hipMemsetAsync(e->r_cursor, 0, sizeof(int) * nc3, st));
     launch_cell_key(e->pos, N, e->prm.box_size, e->nc, e->r_key, e->r_counts, st);
-    launch_scan(e->r_counts, e->r_off, nc3, e->scan_sums, st);
+    if (launch_scan(e->r_counts, e->r_off, nc3, e->scan_sums, st))
+      return fail(-1, "scan of more than 2^22 values");
     launch_pscatter(e->r_key, e->r_off, e->r_cursor, e->r_perm, N, st);
     if (launch_bucket_sort(e->r_key, N, e->r_off, e->r_perm, nc3, e->bigq, e->r_iperm, st))
       return fail(-2, "cell sort launch failed");
@@ -444,7 +446,8 @@ int build_P(_spme_t *e, hipStream_t st) {
                   "real-space block list needs %ld entries (at most %d): the particles are "
                   "too clustered for rc = %g",
                   total, INT_MAX, rc);
-    launch_scan(e->r_size, e->r_soff, e->nslices, e->scan_sums, st);
+    if (launch_scan(e->r_size, e->r_soff, e->nslices, e->scan_sums, st))
+      return fail(-1, "scan of more than 2^22 values");
     if (total > e->r_cap) {
       (void)hipFree(e->r_col);
       (void)hipFree(e->r_blk);

@@ -470,14 +470,16 @@ void launch_pbuild(const double *pos, int N, double L, int K, int P, const WTabl
 
 int scan_tiles(int n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
-// sums: scan_tiles(n) ints of scratch
-void launch_scan(const int *in, int *out, int n, int *sums, hipStream_t st) {
+// sums: scan_tiles(n) <= 1024 ints of scratch; -1 if n is too large
+int launch_scan(const int *in, int *out, int n, int *sums, hipStream_t st) {
   const int nt = scan_tiles(n);
+  if (nt > 1024) return -1;
   if (nt > 0)
     hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(1024), 0, st, in, n, sums);
   hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, sums, nt, out, n);
   if (nt > 0)
     hipLaunchKernelGGL(k_scan_out, dim3(nt), dim3(1024), 0, st, in, n, sums, out);
+  return 0;
 }
 
 void launch_pscatter(const int *key, const int *boff, int *cursor, int *perm, int N,

@@ -134,7 +134,7 @@ void launch_pbuild(const double *pos, int N, double L, int K, int P, const WTabl
                    int TY, int *base_tmp, double *w_tmp, int *key_tmp, int *counts, int *err,
                    hipStream_t st);
 // Exclusive scan of n <= 2^22 ints, out[n] = total; sums: 1024 ints of scratch.
-void launch_scan(const int *in, int *out, int n, int *sums, hipStream_t st);
+int launch_scan(const int *in, int *out, int n, int *sums, hipStream_t st);
 void launch_pscatter(const int *key, const int *boff, int *cursor, int *perm, int N,
                      hipStream_t st);
 // buckets above SORT_MAX particles are rebuilt by a stable pass over the keys

@@ -182,11 +182,8 @@ __device__ __forceinline__ int cell_of(double p, double L, int nc) {
 
 // fn(j, ex, ey, ez, r2) for every j != s within the cutoff, over the 27
 // cells around s's in a fixed order (dz, dy, dx ascending, j ascending
-// within a cell).  The three cells of a (dz, dy) row that do not wrap in x
-// are one contiguous range of the cell order.  Candidates are read four at
-// a time, so their position loads are in flight together.  The distance
-// test is evaluated without contraction, so the count and column passes
-// (separate kernels) decide alike.
+// within a cell).  The distance test is evaluated without contraction, so
+// the count and column passes (separate kernels) decide alike.
 template <typename F>
 __device__ __forceinline__ void for_neighbours(const double *__restrict__ spos,
                                                const int *__restrict__ coff, int nc, double L,
@@ -194,45 +191,26 @@ __device__ __forceinline__ void for_neighbours(const double *__restrict__ spos,
   const double px = spos[3 * s], py = spos[3 * s + 1], pz = spos[3 * s + 2];
   const int cx0 = cell_of(px, L, nc), cy0 = cell_of(py, L, nc), cz0 = cell_of(pz, L, nc);
   const double iL = 1.0 / L;
-  auto scan_range = [&](int j0, int j1) {
-    for (int jb = j0; jb < j1; jb += 4) {
-      double qx[4], qy[4], qz[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int j = min(jb + u, j1 - 1);
-        qx[u] = spos[3 * j];
-        qy[u] = spos[3 * j + 1];
-        qz[u] = spos[3 * j + 2];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-#pragma clang fp contract(off)
-        const int j = jb + u;
-        if (j >= j1 || j == s) continue;
-        double ex = px - qx[u], ey = py - qy[u], ez = pz - qz[u];
-        ex -= L * rint(ex * iL);
-        ey -= L * rint(ey * iL);
-        ez -= L * rint(ez * iL);
-        const double r2 = ex * ex + ey * ey + ez * ez;
-        if (r2 >= cut2 || r2 == 0.0) continue;
-        fn(j, ex, ey, ez, r2);
-      }
-    }
-  };
   for (int dz = -1; dz <= 1; ++dz) {
     int cz = cz0 + dz;
     cz += cz < 0 ? nc : (cz >= nc ? -nc : 0);
     for (int dy = -1; dy <= 1; ++dy) {
       int cy = cy0 + dy;
       cy += cy < 0 ? nc : (cy >= nc ? -nc : 0);
-      const int row = (cz * nc + cy) * nc;
-      if (cx0 >= 1 && cx0 <= nc - 2) {
-        scan_range(coff[row + cx0 - 1], coff[row + cx0 + 2]);
-      } else {
-        for (int dx = -1; dx <= 1; ++dx) {
-          int cx = cx0 + dx;
-          cx += cx < 0 ? nc : (cx >= nc ? -nc : 0);
-          scan_range(coff[row + cx], coff[row + cx + 1]);
+      for (int dx = -1; dx <= 1; ++dx) {
+        int cx = cx0 + dx;
+        cx += cx < 0 ? nc : (cx >= nc ? -nc : 0);
+        const int cell = (cz * nc + cy) * nc + cx;
+        for (int j = coff[cell], j1 = coff[cell + 1]; j < j1; ++j) {
+#pragma clang fp contract(off)
+          if (j == s) continue;
+          double ex = px - spos[3 * j], ey = py - spos[3 * j + 1], ez = pz - spos[3 * j + 2];
+          ex -= L * rint(ex * iL);
+          ey -= L * rint(ey * iL);
+          ez -= L * rint(ez * iL);
+          const double r2 = ex * ex + ey * ey + ez * ez;
+          if (r2 >= cut2 || r2 == 0.0) continue;
+          fn(j, ex, ey, ez, r2);
         }
       }
     }
