@@ -220,6 +220,19 @@ long spme_block_gram_noalloc_max(void);
 int spme_block_gram_noalloc(int precision, long n, int sx, int sy, const void *X, long ldx,
                             const void *Y, long ldy, double *C, void *stream);
 
+/* No counterpart in the reference: a measurement aid.  Streaming probes give
+ * the HBM rates that plain hand-written streaming kernels reach on the
+ * current GPU, the practical ceilings beside the published peak.  mode 0
+ * reads a (b[k] = sum of what block k read, for k < blocks), 1 fills b with
+ * value, 2 copies a to b, 3 scales b in place by value.  bytes per buffer (a
+ * positive multiple of 16), device buffers 16-byte aligned and not
+ * overlapping; blocks of 256 threads in a grid-stride loop (0: 16 per
+ * compute unit); nontemporal != 0 uses non-temporal loads and stores.  One
+ * untimed launch, then reps timed ones on stream (NULL: the default stream);
+ * *ms = the mean time of one launch.  Synchronous. */
+int spme_stream_probe(int mode, int nontemporal, const void *a, void *b, long bytes,
+                      float value, int blocks, int reps, void *stream, float *ms);
+
 /* Layout and sizes, for tests and benchmarks. */
 typedef struct spme_info {
   int K, P, N, precision, max_rhs;

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspme_hip.so")
 SOURCES = ["spme_group.hip", "spme_kernels.hip", "spme_real.hip", "spme_engine.hip",
-           "spme_mlp.hip", "spme_dense.hip", "spme_block.hip"]
+           "spme_mlp.hip", "spme_dense.hip", "spme_block.hip", "spme_stream.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SPME_OFFLOAD_ARCH", "gfx950")
 

@@ -40,6 +40,30 @@ def _torch():
     return torch
 
 
+STREAM_MODES = {"read": 0, "write": 1, "copy": 2, "scale": 3}
+# HBM bytes one launch moves, per buffer byte
+STREAM_TRAFFIC = {"read": 1, "write": 1, "copy": 2, "scale": 2}
+
+
+def stream_probe(mode, a, b, value=1.0, nontemporal=False, blocks=0, reps=5):
+    """Mean ms of one launch of a hand-written streaming kernel (no
+    counterpart in the reference): 'read' a (b[k] gets block k's sum),
+    'write' b = value, 'copy' b = a, 'scale' b *= value.  a and b: device
+    tensors of the same byte size (a multiple of 16); runs on torch's current
+    stream, synchronously."""
+    torch = _torch()
+    nbytes = b.numel() * b.element_size()
+    if a is not None and a.numel() * a.element_size() != nbytes:
+        raise SpmeError("stream_probe: a and b differ in size")
+    ms = ctypes.c_float()
+    stream = torch.cuda.current_stream().cuda_stream
+    check(load().spme_stream_probe(STREAM_MODES[mode], int(bool(nontemporal)),
+                                   None if a is None else a.data_ptr(), b.data_ptr(),
+                                   nbytes, float(value), int(blocks), int(reps), stream,
+                                   ctypes.byref(ms)), "spme_stream_probe")
+    return ms.value
+
+
 class Engine:
     """One SPME engine (positions, P data, mesh) on the current GPU."""
 

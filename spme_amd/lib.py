@@ -27,6 +27,7 @@ EXPORTED = [
     "spme_group_get_tiles", "spme_group_autotune", "spme_group_apply",
     "spme_block_gram_worksize", "spme_block_gram", "spme_block_update",
     "spme_block_gram_noalloc_max", "spme_block_gram_noalloc", "spme_engine_set_radii",
+    "spme_stream_probe",
 ]
 
 TILE_FIELDS = ["spread_cc", "spread_zc", "interp_zc", "spread_persist",
@@ -122,6 +123,8 @@ def load() -> ctypes.CDLL:
     L.spme_block_gram_noalloc_max.argtypes = []
     L.spme_block_gram_noalloc_max.restype = l
     L.spme_block_gram_noalloc.argtypes = [i, l, i, i, vp, l, vp, l, vp, vp]
+    L.spme_stream_probe.argtypes = [i, i, vp, vp, l, ctypes.c_float, i, i, vp,
+                                    ctypes.POINTER(ctypes.c_float)]
     _lib = L
     return L
 
