@@ -36,6 +36,30 @@ __device__ __forceinline__ void permute_elem_cell(const int *__restrict__ perm,
   fs[t] = in[r * ldin + 3L * perm[s] + d];
 }
 
+// Non-temporal store of a 4-, 8- or 16-byte value.  The forward plane pass
+// and the z pass write the half spectrum this way: the next pass reads it
+// back from HBM in any case (the mesh is 1.6 GB at 16 RHS and K = 192), and
+// the hint keeps the written lines from displacing what the pass still
+// reads (measured at K = 192, FP32: 0.676 -> 0.636 ms for the z pass,
+// 0.677 -> 0.666 ms for the forward plane pass).  The inverse plane pass
+// and spreading keep ordinary stores (there the hint cost 0.04 and 1.2 ms).
+template <typename V>
+__device__ __forceinline__ void st_stream(V *p, const V &v) {
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  typedef float v2 __attribute__((ext_vector_type(2)));
+  if constexpr (sizeof(V) == 16) {
+    v4 t;
+    __builtin_memcpy(&t, &v, 16);
+    __builtin_nontemporal_store(t, reinterpret_cast<v4 *>(p));
+  } else if constexpr (sizeof(V) == 8) {
+    v2 t;
+    __builtin_memcpy(&t, &v, 8);
+    __builtin_nontemporal_store(t, reinterpret_cast<v2 *>(p));
+  } else {
+    __builtin_nontemporal_store(v, p);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Spread (old-version/spme_kernel.c:231-312): grid[c] = sum_i q_i (x) f_ic.
 // One wave owns a TX x TY column of cells over a z range and keeps a window
@@ -996,11 +1020,12 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
     reg_fft<N1, +1>(a);
     if (nq) {
 #pragma unroll
-      for (int n1 = 0; n1 < N1; ++n1) nq[c * NQC + (long)(n2 + N2 * n1) * K + y] = a[n1];
+      for (int n1 = 0; n1 < N1; ++n1)
+        st_stream(nq + c * NQC + (long)(n2 + N2 * n1) * K + y, a[n1]);
     } else if (x < Kh) {
 #pragma unroll
       for (int n1 = 0; n1 < N1; ++n1)
-        g0[c * cs + ((size_t)(n2 + N2 * n1) * K + y) * Khp + x] = a[n1];
+        st_stream(g0 + c * cs + ((size_t)(n2 + N2 * n1) * K + y) * Khp + x, a[n1]);
     }
   }
 }
@@ -1394,7 +1419,8 @@ __device__ __forceinline__ void xy_fwd_block(T *__restrict__ grid, int RS, long 
         for (int n2 = 0; n2 < N2; ++n2) bb[n2] = buf[(k1 * N2 + n2) * Kh + x];
         reg_fft<N2, -1>(bb);
 #pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) spec[(size_t)(k1 + N1 * k2) * Khp + x] = bb[k2];
+        for (int k2 = 0; k2 < N2; ++k2)
+          st_stream(spec + (size_t)(k1 + N1 * k2) * Khp + x, bb[k2]);
       }
       if (NYQ && tid < N1) {
         cpx<T> *nq = nyq + ((size_t)comp * K + z) * K;
@@ -1404,7 +1430,7 @@ __device__ __forceinline__ void xy_fwd_block(T *__restrict__ grid, int RS, long 
         for (int n2 = 0; n2 < N2; ++n2) bb[n2] = buf[(k1 * N2 + n2) * Kh + K / 2];
         reg_fft<N2, -1>(bb);
 #pragma unroll
-        for (int k2 = 0; k2 < N2; ++k2) nq[k1 + N1 * k2] = bb[k2];
+        for (int k2 = 0; k2 < N2; ++k2) st_stream(nq + k1 + N1 * k2, bb[k2]);
       }
     }
     __syncthreads();
