@@ -908,6 +908,8 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
   using Z = ZLayout<XW>;
   static_assert(XW * N1 <= NT, "one step-B column per thread");
   constexpr int NXC = Z::chunks(K), PER = Z::per_rhs(K);
+  // with narrow Nyquist blocks every lane's column has x <= K/2 < Kh
+  constexpr bool ALL_X = Z::nyq(K);
   load_tw<T, K>(s_tw, tw);
   __syncthreads();  // step A reads twiddles that other threads loaded
   const int rhs = lb / PER, j = lb - rhs * PER;
@@ -931,8 +933,8 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
 #pragma unroll
     for (int n1 = 0; n1 < N1; ++n1)
       a[n1] = nq ? nq[c * NQC + (long)(n1 * N2 + n2) * K + y]
-                 : (x < Kh ? g0[c * cs + ((size_t)(n1 * N2 + n2) * K + y) * Khp + x]
-                           : cpx<T>{T(0), T(0)});
+                 : (ALL_X || x < Kh ? g0[c * cs + ((size_t)(n1 * N2 + n2) * K + y) * Khp + x]
+                                    : cpx<T>{T(0), T(0)});
     reg_fft<N1, -1>(a);
 #pragma unroll
     for (int k1 = 1; k1 < N1; ++k1) a[k1] = cmul(a[k1], twiddle<-1>(s_tw, n2 * k1));
@@ -1022,7 +1024,7 @@ __device__ __forceinline__ void fft_z_infl_block(T *__restrict__ grid, long comp
 #pragma unroll
       for (int n1 = 0; n1 < N1; ++n1)
         st_stream(nq + c * NQC + (long)(n2 + N2 * n1) * K + y, a[n1]);
-    } else if (x < Kh) {
+    } else if (ALL_X || x < Kh) {
 #pragma unroll
       for (int n1 = 0; n1 < N1; ++n1)
         st_stream(g0 + c * cs + ((size_t)(n2 + N2 * n1) * K + y) * Khp + x, a[n1]);

@@ -6,9 +6,12 @@ namespace spme {
 
 template <typename T> struct cpx { T x, y; };
 
+// The FFT arithmetic states its fused multiply-adds (contraction off), so
+// that every kernel that inlines it rounds alike, whatever the code around.
 template <typename T>
 __device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
-  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+#pragma clang fp contract(off)
+  return {fma(a.x, b.x, -(a.y * b.y)), fma(a.x, b.y, a.y * b.x)};
 }
 template <typename T>
 __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
@@ -37,10 +40,11 @@ __device__ __forceinline__ void small_dft(cpx<T> *v) {
     v[1] = cadd(d02, d13);
     v[3] = csub(d02, d13);
   } else if constexpr (R == 3) {
+#pragma clang fp contract(off)
     const T h = T(0.86602540378443864676372317075294);  // sqrt(3)/2
     cpx<T> t = cadd(v[1], v[2]);
     cpx<T> u = csub(v[1], v[2]);
-    cpx<T> m = {v[0].x - T(0.5) * t.x, v[0].y - T(0.5) * t.y};
+    cpx<T> m = {fma(T(-0.5), t.x, v[0].x), fma(T(-0.5), t.y, v[0].y)};
     cpx<T> r = mul_si<S>(cpx<T>{h * u.x, h * u.y});
     v[0] = cadd(v[0], t);
     v[1] = cadd(m, r);

@@ -343,11 +343,11 @@ __global__ __launch_bounds__(NT) void k_fft_z(T *__restrict__ grid, long comp_st
                                      xcd_remap(blockIdx.x, gridDim.x), buf, s_tw);
 }
 
+// 512-thread blocks are held to 128 VGPRs, so that two blocks fit a CU
 template <typename T, int K, int XW, int NT, bool POLY = false>
-__global__ __launch_bounds__(NT) void k_fft_z_infl(T *__restrict__ grid, long comp_stride,
-                                                   int Khp, const T *__restrict__ tab, T dk,
-                                                   const cpx<T> *__restrict__ tw,
-                                                   cpx<T> *__restrict__ nyq) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 512 ? 4 : 1)))
+void k_fft_z_infl(T *__restrict__ grid, long comp_stride, int Khp, const T *__restrict__ tab,
+                  T dk, const cpx<T> *__restrict__ tw, cpx<T> *__restrict__ nyq) {
   __shared__ cpx<T> buf[(POLY ? 6 : 3) * XW * (K + 1)];
   __shared__ cpx<T> s_tw[K];
   fft_z_infl_block<T, K, XW, NT, POLY>(grid, comp_stride, Khp, tab, dk, tw,
