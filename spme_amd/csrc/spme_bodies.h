@@ -254,9 +254,12 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
     // planes zs + st + j, j < P, held in acc[(st + j) % P]; after the step,
     // plane zs + st (slot st % P) is complete: it is written and its slot
     // cleared for plane zs + st + P.  The steps are unrolled by P so that
-    // every slot index and staging buffer is a constant.
-    auto step = [&](int st, auto uc) {
+    // every slot index and staging buffer is a constant.  In the first P
+    // steps of a sweep (PRO) the planes below the sweep, which another block
+    // or sweep stores, are not accumulated: step u adds to planes j >= P-1-u.
+    auto step_any = [&](int st, auto uc, auto pc) {
       constexpr int u = decltype(uc)::value;  // st % P
+      constexpr int JMIN = decltype(pc)::value ? P - 1 - u : 0;
       constexpr int B = (u / G) & 1;          // this group's buffer
       const std::integral_constant<int, B> bc{};
       const int *spb = B ? s_pb1 : s_pb0;
@@ -279,7 +282,7 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
 #pragma unroll
           for (int c = 0; c < CC; ++c) wf[c] = wxy * sf[k * CC + c];
 #pragma unroll
-          for (int j = 0; j < P; ++j) {
+          for (int j = JMIN; j < P; ++j) {
             const T wz = w[2 * P + j];
 #pragma unroll
             for (int c = 0; c < CC; ++c) acc[(u + j) % P][c] += wz * wf[c];
@@ -333,7 +336,10 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
 #pragma unroll
       for (int c = 0; c < CC; ++c) acc[u][c] = T(0);
     };
-    for (int st0 = 0; st0 < nsteps; st0 += P) unroll_steps<0, P>(st0, nsteps, step);
+    auto step = [&](int st, auto uc) { step_any(st, uc, std::false_type{}); };
+    auto step_pro = [&](int st, auto uc) { step_any(st, uc, std::true_type{}); };
+    unroll_steps<0, P>(0, nsteps, step_pro);
+    for (int st0 = P; st0 < nsteps; st0 += P) unroll_steps<0, P>(st0, nsteps, step);
   }
   wait_vm_all();
 }

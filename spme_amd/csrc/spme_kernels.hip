@@ -282,7 +282,7 @@ __global__ __launch_bounds__(256) void k_invert(const int *__restrict__ perm,
 
 // Spread: single-system kernel (logical blocks XCD-remapped).
 template <typename T, int P, int CC, int TX, int TY>
-__global__ __launch_bounds__(64) void k_spread(const int *__restrict__ boff,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1))) void k_spread(const int *__restrict__ boff,
                                                const int *__restrict__ pbase,
                                                const T *__restrict__ pw,
                                                const T *__restrict__ fs, int C,
