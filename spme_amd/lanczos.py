@@ -245,6 +245,10 @@ class BlockLanczos:
                     change = float(np.linalg.norm(coef - pad) / np.linalg.norm(coef))
                 coef_prev = coef
                 next_check = m + (1 if m < 8 else max(1, m // 4))
+                # a check due within m/8 steps of the cap moves to the cap,
+                # which is checked anyway
+                if next_check < self.max_iter and self.max_iter - next_check < m // 8:
+                    next_check = self.max_iter
                 if change < self.tol or m == self.max_iter:
                     break
             Qn, B = self._orth(W, scale, basis[rows:rows + kj])
