@@ -268,25 +268,42 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
       // the wave runs as many rounds as its busiest lane has particles, each
       // lane taking its own next covering particle (ascending)
       auto run = [&](Mask m) {
-        while (m) {
+        auto take = [&]() {
           const int k = sizeof(Mask) == 8 ? __builtin_ctzll(m) : __builtin_ctz(unsigned(m));
           m &= m - 1;
+          return k;
+        };
+        auto wxy_of = [&](int k) {
           const int pb = spb[k];
           int dx = x - (pb & 1023);
           dx += dx < 0 ? K : 0;
           int dy = y - ((pb >> 10) & 1023);
           dy += dy < 0 ? K : 0;
           const T *w = sw + k * NW;
-          const T wxy = w[dx] * w[P + dy];
+          return w[dx] * w[P + dy];
+        };
+        if (!m) return;
+        int k = take();
+        T wxy = wxy_of(k);
+        while (true) {
+          const int kc = k;
+          const T wc = wxy;
+          const bool more = m != 0;
+          if (more) {  // the next particle's base and xy weight, ahead of this one's sums
+            k = take();
+            wxy = wxy_of(k);
+          }
+          const T *w = sw + kc * NW;
           T wf[CC];
 #pragma unroll
-          for (int c = 0; c < CC; ++c) wf[c] = wxy * sf[k * CC + c];
+          for (int c = 0; c < CC; ++c) wf[c] = wc * sf[kc * CC + c];
 #pragma unroll
           for (int j = JMIN; j < P; ++j) {
             const T wz = w[2 * P + j];
 #pragma unroll
             for (int c = 0; c < CC; ++c) acc[(u + j) % P][c] += wz * wf[c];
           }
+          if (!more) break;
         }
       };
       if constexpr (u % G == 0) {
