@@ -143,6 +143,10 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
   const int cc0 = (rest / nz) * CC;
   const unsigned *pw_d = reinterpret_cast<const unsigned *>(pw);
   const unsigned *fs_d = reinterpret_cast<const unsigned *>(fs);
+  // 16-byte staging: weight records of whole 16-byte pieces; force slices
+  // too when the rows (C values) keep them 16-byte aligned
+  constexpr bool W16 = NW * Q % 4 == 0, F16 = CC * Q % 4 == 0;
+  const bool f16ok = C * Q % 4 == 0;
 
   // copy particles [b0, b0 + nb) of the concatenated sorted ranges
   // (lo[t], cnt[t]), t < NTL, into buffer B: base, NW weights and the CC
@@ -163,24 +167,55 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
       return i;
     };
     if (lane < nb) __builtin_amdgcn_global_load_lds(pbase + locate(b0 + lane), spb, 4, 0, 0);
+    // a particle's weights (and, when its rows are 16-byte aligned, its
+    // forces) in 16-byte pieces: a quarter of the loads and index searches
+    // of one dword per lane
+    if constexpr (W16) {
+      constexpr int PW = NW * Q / 4, RW16 = (MAXB * PW + 63) / 64;
 #pragma unroll
-    for (int r = 0; r < RWD; ++r) {
-      const int e = lane + 64 * r;
-      if (e < nb * NW * Q) {
-        const int k = e / (NW * Q);
-        const unsigned off = unsigned(locate(b0 + k)) * unsigned(NW * Q) + unsigned(e - k * NW * Q);
-        __builtin_amdgcn_global_load_lds(pw_d + off, reinterpret_cast<char *>(sw) + 256 * r, 4, 0, 0);
+      for (int r = 0; r < RW16; ++r) {
+        const int e = lane + 64 * r;
+        if (e < nb * PW) {
+          const int k = e / PW;
+          const unsigned off = unsigned(locate(b0 + k)) * unsigned(NW * Q) + unsigned(e - k * PW) * 4u;
+          __builtin_amdgcn_global_load_lds(pw_d + off, reinterpret_cast<char *>(sw) + 1024 * r, 16, 0, 0);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < RWD; ++r) {
+        const int e = lane + 64 * r;
+        if (e < nb * NW * Q) {
+          const int k = e / (NW * Q);
+          const unsigned off = unsigned(locate(b0 + k)) * unsigned(NW * Q) + unsigned(e - k * NW * Q);
+          __builtin_amdgcn_global_load_lds(pw_d + off, reinterpret_cast<char *>(sw) + 256 * r, 4, 0, 0);
+        }
       }
     }
+    if (F16 && f16ok) {
+      constexpr int PF = CC * Q / 4, RF16 = (MAXB * PF + 63) / 64;
 #pragma unroll
-    for (int r = 0; r < RFD; ++r) {
-      const int e = lane + 64 * r;
-      if (e < nb * CC * Q) {
-        const int k = e / (CC * Q);
-        const unsigned off =
-            (unsigned(locate(b0 + k)) * unsigned(C) + unsigned(cc0)) * unsigned(Q) +
-            unsigned(e - k * CC * Q);
-        __builtin_amdgcn_global_load_lds(fs_d + off, reinterpret_cast<char *>(sf) + 256 * r, 4, 0, 0);
+      for (int r = 0; r < RF16; ++r) {
+        const int e = lane + 64 * r;
+        if (e < nb * PF) {
+          const int k = e / PF;
+          const unsigned off =
+              (unsigned(locate(b0 + k)) * unsigned(C) + unsigned(cc0)) * unsigned(Q) +
+              unsigned(e - k * PF) * 4u;
+          __builtin_amdgcn_global_load_lds(fs_d + off, reinterpret_cast<char *>(sf) + 1024 * r, 16, 0, 0);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < RFD; ++r) {
+        const int e = lane + 64 * r;
+        if (e < nb * CC * Q) {
+          const int k = e / (CC * Q);
+          const unsigned off =
+              (unsigned(locate(b0 + k)) * unsigned(C) + unsigned(cc0)) * unsigned(Q) +
+              unsigned(e - k * CC * Q);
+          __builtin_amdgcn_global_load_lds(fs_d + off, reinterpret_cast<char *>(sf) + 256 * r, 4, 0, 0);
+        }
       }
     }
   };
