@@ -150,8 +150,8 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
 
   // copy particles [b0, b0 + nb) of the concatenated sorted ranges
   // (lo[t], cnt[t]), t < NTL, into buffer B: base, NW weights and the CC
-  // forces of this pass, one dword per lane per load (32-bit dword offsets:
-  // the engine bounds N so that N * C values fit)
+  // forces of this pass, 16 bytes or one dword per lane per load (32-bit
+  // dword offsets: the engine bounds N so that N * C values fit)
   auto stage = [&](auto bc, const int (&lo)[NTL], const int (&cnt)[NTL], int b0, int nb) {
     constexpr int B = decltype(bc)::value;
     int *spb = B ? s_pb1 : s_pb0;
@@ -177,7 +177,8 @@ __device__ __forceinline__ void spread_block(const int *__restrict__ boff,
         const int e = lane + 64 * r;
         if (e < nb * PW) {
           const int k = e / PW;
-          const unsigned off = unsigned(locate(b0 + k)) * unsigned(NW * Q) + unsigned(e - k * PW) * 4u;
+          const unsigned off =
+              unsigned(locate(b0 + k)) * unsigned(NW * Q) + unsigned(e - k * PW) * 4u;
           __builtin_amdgcn_global_load_lds(pw_d + off, reinterpret_cast<char *>(sw) + 1024 * r, 16, 0, 0);
         }
       }
